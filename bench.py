@@ -1,0 +1,225 @@
+"""Throughput of the convolution hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+One step is one convolution call over one batch of synthetic input already
+resident in HBM.  The default workload is the 2-D f32 conv_fft:
+x 200x5000, k 11x31, dilation 2, ConvMode::Same,
+PaddingMode::Custom([Reflect, Circular]) (1,000,000 outputs per step).
+
+Prints ONE JSON line with the throughput, the roofline of the dominant kernel
+(its algorithmic bytes per launch over its average duration, timed with HIP
+events on the library's stream in a second pass over the same steps) and a
+CPU baseline (the oracle's float32 restatement of the crate's algorithm with
+scipy's pocketfft, timed on this host on a bounded sample).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ndarray-conv_amd"))
+
+METRIC = "conv outputs/s + % HBM/MFMA roofline; BF16x3 vs FP32 at ≤1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def workloads():
+    import ndconv as nc
+    B = nc.BorderType
+    return {
+        "fft2d": dict(
+            name="2-D f32 conv_fft x200x5000 k11x31 dilation 2 Same Custom[Reflect,Circular]",
+            seed=2, op="fft", dtype=np.float32, x=(200, 5000), k=(11, 31), dil=2, mode=nc.ConvMode.Same,
+            pad=nc.PaddingMode.Custom([B.Reflect, B.Circular]), dist="uniform",
+            cpu=("same", ("custom", ["reflect", "circular"]))),
+        "fft1d": dict(
+            name="1-D f32 conv_fft x5000 k31 Same Zeros", seed=1, op="fft", dtype=np.float32, x=(5000,), k=(31,),
+            dil=1, mode=nc.ConvMode.Same, pad=nc.PaddingMode.Zeros, dist="uniform", cpu=("same", "zeros")),
+        "fft3d": dict(
+            name="3-D f32 conv_fft x10x100x200 k5x11x31 Full Zeros", seed=3, op="fft", dtype=np.float32,
+            x=(10, 100, 200), k=(5, 11, 31), dil=1, mode=nc.ConvMode.Full, pad=nc.PaddingMode.Zeros,
+            dist="uniform", cpu=("full", "zeros")),
+        "fft3d_c64": dict(
+            name="3-D c64 conv_fft x10x100x200 k5x11x31 Full Zeros", seed=3, op="fft", dtype=np.complex64,
+            x=(10, 100, 200), k=(5, 11, 31), dil=1, mode=nc.ConvMode.Full, pad=nc.PaddingMode.Zeros,
+            dist="normal", cpu=("full", "zeros")),
+        "direct2d_i32": dict(
+            name="2-D i32 conv x200x5000 k11x31 Same Custom[Reflect,Circular]", seed=2, op="conv",
+            dtype=np.int32, x=(200, 5000), k=(11, 31), dil=1, mode=nc.ConvMode.Same,
+            pad=nc.PaddingMode.Custom([B.Reflect, B.Circular]), dist="int8", cpu=None),
+        "direct2d_f32": dict(
+            name="2-D f32 conv x4096x4096 k31x31 dilation 2 stride 2 Valid Replicate", seed=4, op="conv",
+            dtype=np.float32, x=(4096, 4096), k=(31, 31), dil=2, mode=nc.ConvMode.Custom([0, 0], [2, 2]),
+            pad=nc.PaddingMode.Replicate, dist="normal", cpu=None),
+    }
+
+
+def make_inputs(w):
+    rng = np.random.default_rng(w["seed"])
+    dt = np.dtype(w["dtype"])
+    if w["dist"] == "uniform":
+        return rng.random(w["x"]).astype(dt), rng.random(w["k"]).astype(dt)
+    if w["dist"] == "int8":
+        return rng.integers(-8, 9, w["x"]).astype(dt), rng.integers(-8, 9, w["k"]).astype(dt)
+    if dt.kind == "c":
+        x = rng.standard_normal(w["x"]) + 1j * rng.standard_normal(w["x"])
+        k = rng.standard_normal(w["k"]) + 1j * rng.standard_normal(w["k"])
+        return x.astype(dt), k.astype(dt)
+    return rng.standard_normal(w["x"]).astype(dt), rng.standard_normal(w["k"]).astype(dt)
+
+
+def cpu_baseline(w, x, k, budget_s=12.0):
+    """The oracle's f32 restatement (scipy pocketfft) on this host's cores."""
+    if w["cpu"] is None:
+        return None
+    from oracle import ndconv_oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    mode, pm = w["cpu"]
+    f = lambda: O.conv_fft(x, k, mode, pm, w["dil"], True, precision="f32", workers=cores)
+    f()
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end and len(times) < 2000:
+        t0 = time.perf_counter()
+        f()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    outs = int(np.prod(f().shape))
+    return {"value": outs / med, "unit": "outputs/s", "cores": cores, "kind": "port",
+            "sample": f"{len(times)} full calls of the same workload (median {med * 1e3:.3f} ms), "
+                      f"oracle float32 restatement of conv_fft with scipy pocketfft, workers={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="fft2d")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+
+    import torch
+    import ndconv as nc
+    from ndconv import _lib as C
+
+    w = workloads()[args.workload]
+    x, k = make_inputs(w)
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(x).to(dev)
+    kt = torch.from_numpy(k).to(dev)
+    lib = C.lib()
+    proc = nc.get_fft_processor()
+    h = proc.handle
+    d = C.ConvDesc()
+    kwd = nc.with_dilation(kt, w["dil"])
+    nc._fill_desc(d, xt, kwd, w["mode"], w["pad"], nc._NP_DTYPE[np.dtype(w["dtype"])], host=False)
+    op = C.OP_CONV_FFT if w["op"] == "fft" else C.OP_CONV
+    info = C.PlanInfo()
+    nc._check(lib.ndc_plan(ctypes.byref(d), op, ctypes.byref(info)))
+    out_shape = tuple(int(info.out_shape[i]) for i in range(len(w["x"])))
+    yt = torch.empty(out_shape, dtype=xt.dtype, device=dev)
+    fn = lib.ndc_conv_fft if op == C.OP_CONV_FFT else lib.ndc_conv
+    stream = torch.cuda.current_stream(dev)
+    nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
+    px, pk, py = ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(kt.data_ptr()), ctypes.c_void_p(yt.data_ptr())
+    dref = ctypes.byref(d)
+
+    def step():
+        st = fn(h, dref, px, pk, py)
+        if st:
+            nc._check(st, h)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    outputs = int(np.prod(out_shape))
+    value = outputs / (ms * 1e-3)
+
+    # per-kernel durations: HIP events around every launch on the same stream
+    nc._check(lib.ndc_profile_enable(h, 1), h)
+    for _ in range(args.steps):
+        step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    nc._check(lib.ndc_profile_report(h, buf, len(buf)), h)
+    nc._check(lib.ndc_profile_enable(h, 0), h)
+    prof = json.loads(buf.value.decode())
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    flops_per_launch = dom["flops"] / dom["launches"]
+    if w["op"] == "fft":
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+    roof["kernel"] = dom_name
+    roof["avg_ms"] = avg_ms
+    roof["bytes_per_launch"] = bytes_per_launch
+    kernels = {n: {"avg_ms": v["total_ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
+                   "GB/s": (v["bytes"] / v["launches"]) / (v["total_ms"] / v["launches"] * 1e-3) / 1e9}
+               for n, v in prof.items()}
+
+    graph_value = None
+    if args.graph:
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(dev)
+        s.wait_stream(stream)
+        with torch.cuda.stream(s):
+            nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(s.cuda_stream)), h)
+            step()
+            torch.cuda.synchronize()
+            with torch.cuda.graph(g, stream=s):
+                step()
+        torch.cuda.synchronize()
+        for _ in range(3):
+            g.replay()
+        torch.cuda.synchronize()
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g0.record()
+        for _ in range(args.steps):
+            g.replay()
+        g1.record()
+        torch.cuda.synchronize()
+        graph_value = outputs / (g0.elapsed_time(g1) / args.steps * 1e-3)
+        nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
+
+    cpu = None if args.no_cpu else cpu_baseline(w, x, k)
+    line = {"metric": METRIC, "value": value, "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": {np.dtype(np.float32): "f32", np.dtype(np.complex64): "c64",
+                                           np.dtype(np.int32): "i32"}[np.dtype(w["dtype"])],
+            "data": "synthetic", "config": {"workload": w["name"], "outputs_per_step": outputs,
+                                            "seed": w["seed"]},
+            "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall}
+    if graph_value is not None:
+        line["graph_replay_value"] = graph_value
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,211 @@
+/*
+ * ndconv.h - C ABI of the MI355X (gfx950) convolution library.
+ *
+ * This is the drop-in boundary for ndarray-conv's convolution hot path: the
+ * Rust crate's trait methods bind to these entry points through a thin FFI
+ * shim.  Every declaration names the reference interface it replaces
+ * (paths relative to the ndarray-conv 0.6.1 source tree).
+ *
+ * Conventions
+ *  - Plain C: pointers, sizes, enums and POD structs only.
+ *  - Element strides are in elements and may be negative; a data pointer
+ *    always points at the first logical element (ndarray's as_ptr()).
+ *  - Outputs are compact C-order arrays allocated by the caller; query the
+ *    shape with ndc_plan().
+ *  - By default x, k and y are device (HBM) pointers and the call is
+ *    asynchronous on the handle's stream.  With NDC_FLAG_HOST_MEMORY they
+ *    are host pointers and the call is synchronous.
+ *  - A handle is not thread-safe; use one per thread, as the reference's
+ *    &mut Processor is used.
+ */
+#ifndef NDCONV_H
+#define NDCONV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NDC_MAX_RANK 6
+#define NDC_ABI_VERSION 1
+
+/* Error<N> (lib.rs:145-159); reference panics map to NDC_ERR_INVALID_ARGUMENT. */
+typedef enum ndc_status {
+    NDC_OK = 0,
+    NDC_ERR_DATA_SHAPE = 1,       /* Error::DataShape      (lib.rs:151-152) */
+    NDC_ERR_KERNEL_SHAPE = 2,     /* Error::KernelShape    (lib.rs:154-155) */
+    NDC_ERR_MISMATCH_SHAPE = 3,   /* Error::MismatchShape  (lib.rs:157-158) */
+    NDC_ERR_INVALID_ARGUMENT = 4, /* inputs the reference panics on: stride 0, reflect
+                                     pads leaving the axis (half_dim.rs:200,247),
+                                     FFT dilation 0; or a malformed descriptor */
+    NDC_ERR_UNSUPPORTED = 5,      /* dtype/algorithm combination not provided */
+    NDC_ERR_NO_DEVICE = 6,        /* no HIP device visible */
+    NDC_ERR_DEVICE = 7,           /* a HIP runtime call failed */
+    NDC_ERR_OUT_OF_MEMORY = 8
+} ndc_status;
+
+/* Element types: T of ConvExt (any NumAssign + Copy, conv/mod.rs:118-127) and
+ * InElem of ConvFFTExt (f32/f64 or Complex<f32/f64>, conv_fft/mod.rs:81-87). */
+typedef enum ndc_dtype {
+    NDC_F32 = 0, NDC_F64 = 1, NDC_C64 = 2, NDC_C128 = 3,
+    NDC_I8 = 4, NDC_I16 = 5, NDC_I32 = 6, NDC_I64 = 7,
+    NDC_U8 = 8, NDC_U16 = 9, NDC_U32 = 10, NDC_U64 = 11
+} ndc_dtype;
+
+/* ConvMode<N> (lib.rs:80-105) */
+typedef enum ndc_conv_mode_tag {
+    NDC_CONV_FULL = 0, NDC_CONV_SAME = 1, NDC_CONV_VALID = 2,
+    NDC_CONV_CUSTOM = 3, NDC_CONV_EXPLICIT = 4
+} ndc_conv_mode_tag;
+
+typedef struct ndc_conv_mode {
+    int32_t tag;                        /* ndc_conv_mode_tag */
+    int32_t reserved;
+    int64_t padding[NDC_MAX_RANK][2];   /* Custom: [i][0] is used for both sides; Explicit: per side */
+    int64_t strides[NDC_MAX_RANK];      /* Custom / Explicit only */
+} ndc_conv_mode;
+
+/* A scalar of the element type: re (and im for complex) for floats, i for integers
+ * (two's complement bits for unsigned types). */
+typedef struct ndc_value {
+    double re;
+    double im;
+    int64_t i;
+} ndc_value;
+
+/* BorderType<T> (lib.rs:129-143) */
+typedef enum ndc_border_tag {
+    NDC_BORDER_ZEROS = 0, NDC_BORDER_CONST = 1, NDC_BORDER_REFLECT = 2,
+    NDC_BORDER_REPLICATE = 3, NDC_BORDER_CIRCULAR = 4
+} ndc_border_tag;
+
+typedef struct ndc_border {
+    int32_t tag;         /* ndc_border_tag */
+    int32_t reserved;
+    ndc_value value;     /* Const(c) */
+} ndc_border;
+
+/* PaddingMode<N, T> (lib.rs:106-127) */
+typedef enum ndc_padding_tag {
+    NDC_PAD_ZEROS = 0, NDC_PAD_CONST = 1, NDC_PAD_REFLECT = 2, NDC_PAD_REPLICATE = 3,
+    NDC_PAD_CIRCULAR = 4, NDC_PAD_CUSTOM = 5, NDC_PAD_EXPLICIT = 6
+} ndc_padding_tag;
+
+typedef struct ndc_padding_mode {
+    int32_t tag;                               /* ndc_padding_tag */
+    int32_t reserved;
+    ndc_value value;                           /* Const(c) */
+    ndc_border borders[NDC_MAX_RANK][2];       /* Custom: [i][0]; Explicit: [i][front/back] */
+} ndc_padding_mode;
+
+/* Algorithm selection for ndc_conv. */
+typedef enum ndc_algo {
+    NDC_ALGO_AUTO = 0,          /* exact direct convolution in the element type */
+    NDC_ALGO_DIRECT = 1,        /* same as AUTO */
+    NDC_ALGO_MFMA_BF16 = 2,     /* f32 only, rank 2: one BF16 product per tap on the
+                                   matrix cores, FP32 accumulation */
+    NDC_ALGO_MFMA_BF16X3 = 3    /* f32 only, rank 2: split hi/lo BF16, three products per
+                                   tap, FP32 accumulation (about 16 mantissa bits per product) */
+} ndc_algo;
+
+enum {
+    NDC_FLAG_HOST_MEMORY = 1u,  /* x, k, y are host pointers; the call is synchronous */
+    NDC_FLAG_X_STRIDED = 2u,    /* x_strides is valid (otherwise x is C-contiguous) */
+    NDC_FLAG_K_STRIDED = 4u     /* k_strides is valid (otherwise k is C-contiguous) */
+};
+
+/* One convolution: the arguments of ConvExt::conv / ConvFFTExt::conv_fft.
+ * kernel + dilation + reverse stand for KernelWithDilation (dilation/mod.rs:8-12):
+ * `&kernel` is dilation 1 with reverse = 1 (dilation/mod.rs:196-203); no_reverse()
+ * sets reverse = 0 (dilation/mod.rs:181-188). */
+typedef struct ndc_conv_desc {
+    int32_t dtype;                       /* ndc_dtype of x, k and y */
+    int32_t rank;                        /* N, 1..NDC_MAX_RANK */
+    int64_t x_shape[NDC_MAX_RANK];
+    int64_t x_strides[NDC_MAX_RANK];
+    int64_t k_shape[NDC_MAX_RANK];
+    int64_t k_strides[NDC_MAX_RANK];
+    int64_t dilation[NDC_MAX_RANK];
+    int32_t reverse;
+    int32_t algo;                        /* ndc_algo (ndc_conv only) */
+    ndc_conv_mode conv_mode;
+    ndc_padding_mode padding_mode;
+    uint32_t flags;
+    uint32_t reserved;
+} ndc_conv_desc;
+
+typedef enum ndc_op {
+    NDC_OP_CONV = 0,        /* direct convolution (conv/mod.rs:128-200) */
+    NDC_OP_CONV_FFT = 1     /* FFT convolution (conv_fft/mod.rs:185-292) */
+} ndc_op;
+
+/* Shape arithmetic of one convolution. */
+typedef struct ndc_plan_info {
+    int64_t padding[NDC_MAX_RANK][2];   /* ConvMode::unfold (conv/mod.rs:28-66) */
+    int64_t strides[NDC_MAX_RANK];
+    int64_t kernel_dilated[NDC_MAX_RANK];/* k*d - d + 1 (conv/mod.rs:146-147) */
+    int64_t padded[NDC_MAX_RANK];       /* n + p0 + p1 (conv_fft/mod.rs:220-221) */
+    int64_t fft_size[NDC_MAX_RANK];     /* good_size::compute (good_size.rs:40-42); 0 for NDC_OP_CONV */
+    int64_t out_shape[NDC_MAX_RANK];    /* output shape (conv/mod.rs:162-167, conv_fft/mod.rs:282-289) */
+} ndc_plan_info;
+
+typedef struct ndc_handle_s* ndc_handle;
+
+/* Library identification.  ndc_struct_size lets an FFI binding check its
+ * struct layouts: 0 ndc_conv_desc, 1 ndc_plan_info, 2 ndc_conv_mode,
+ * 3 ndc_padding_mode, 4 ndc_border, 5 ndc_value; -1 for anything else.
+ * (No reference counterpart: FFI hygiene for the shims.) */
+int32_t ndc_abi_version(void);
+int64_t ndc_struct_size(int32_t which);
+const char* ndc_status_string(ndc_status s);
+
+/* Handle lifecycle: stands in for the FFT processor (get_fft_processor,
+ * processor/mod.rs:71-73; GetProcessor::get_processor, processor/mod.rs:125-143):
+ * it owns the HIP stream, the device workspace, twiddle tables and plan caches,
+ * which is what conv_fft_with_processor reuses across calls (conv_fft/mod.rs:151-157). */
+ndc_status ndc_create(ndc_handle* out);
+ndc_status ndc_destroy(ndc_handle h);
+/* Use an external HIP stream (hipStream_t; NULL is the null stream) for all
+ * later calls; ndc_reset_stream returns to the handle's own stream. */
+ndc_status ndc_set_stream(ndc_handle h, void* hip_stream);
+ndc_status ndc_reset_stream(ndc_handle h);
+ndc_status ndc_synchronize(ndc_handle h);
+/* Detail message of the last failed call on this handle. */
+const char* ndc_last_error(ndc_handle h);
+
+/* Validates a descriptor and returns its shape arithmetic, with the
+ * reference's error precedence (conv/mod.rs:136-158; conv_fft/mod.rs:205-227). */
+ndc_status ndc_plan(const ndc_conv_desc* desc, int32_t op, ndc_plan_info* info);
+
+/* Resolved border map of one axis: for each padded position (length padded[axis]
+ * for NDC_OP_CONV, fft_size[axis] for NDC_OP_CONV_FFT) the source index into x
+ * (>= 0) or a code: -1 never written (initial value), -2 the front side's
+ * constant, -3 the back side's constant, -4 outside the padded region (zero).
+ * This is the sequential per-axis fill of padding/mod.rs:119-153 and
+ * padding/half_dim.rs:30-343 restated as an index map. */
+ndc_status ndc_border_map(const ndc_conv_desc* desc, int32_t op, int32_t axis, int32_t* codes, int64_t len);
+
+/* ConvExt::conv (conv/mod.rs:110-115, impl :128-200). */
+ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
+
+/* ConvFFTExt::conv_fft / conv_fft_with_processor / conv_fft_par
+ * (conv_fft/mod.rs:113-171, impls :394-423).  The three differ only in CPU
+ * threading and planner reuse, which the handle subsumes. */
+ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
+
+/* Per-kernel profile (no reference counterpart: the reference has no
+ * instrumentation).  While enabled, every kernel launch on the handle is
+ * bracketed by HIP events on the handle's stream.  The report synchronizes
+ * the stream and writes a JSON object
+ *   {"<kernel>": {"launches": n, "total_ms": t, "bytes": b, "flops": f}, ...}
+ * where bytes/flops are the summed algorithmic HBM traffic and arithmetic of
+ * those launches; then it clears the records. */
+ndc_status ndc_profile_enable(ndc_handle h, int32_t on);
+ndc_status ndc_profile_report(ndc_handle h, char* buf, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NDCONV_H */

@@ -1,0 +1,305 @@
+// Direct convolution on gfx950: out[o] = sum_t pds[o*s + t*d] * kv(t) over the
+// non-zero taps of the (reversed) kernel (conv/mod.rs:177-197,
+// dilation/mod.rs:34-60), with the padded input never materialised: each
+// workgroup stages the input region its output tile needs into LDS, resolving
+// every padded coordinate through the per-axis border maps on the way in.
+//
+// Layout: the last two axes form the tile plane (a rank-1 problem gets a
+// virtual unit axis in front); lower axes are looped over, one kernel tap at
+// a time.  Arithmetic stays in the element type (integers wrap).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace ndc {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTileX = 64;           // outputs per tile row: one wavefront
+constexpr int kTileY = 16;           // tile rows
+constexpr int kItems = kTileY * kTileX / kThreads;
+
+template <typename T>
+struct DirectArgs {
+    const T* x;
+    const T* k;
+    T* y;
+    const int32_t* map[kMaxRank];
+    T cst[kMaxRank][2];
+    int64_t xstr[kMaxRank];
+    int64_t kstr[kMaxRank];
+    int64_t s[kMaxRank];
+    int64_t d[kMaxRank];
+    int64_t out[kMaxRank];
+    int64_t K[kMaxRank];
+    int64_t nouter;     // product of the output extents of axes 0..No-3
+    int64_t ktaps_outer;// product of the kernel extents of axes 0..No-3
+    int32_t No;         // rank after promotion to >= 2
+    int32_t rev;
+    int32_t KYc, KXc;   // kernel chunk staged per pass
+    int32_t gy, gx;     // gcd(stride, dilation) per plane axis: region subsampling
+    int32_t RY, RX;     // region extents in LDS
+    int32_t ntx, nty;   // tiles per plane
+};
+
+template <typename T>
+__device__ __forceinline__ T resolve_plane(const DirectArgs<T>& a, int32_t cy, int32_t cx, int oc_axis, T ocval,
+                                           bool ounw, int64_t xoff) {
+    const int ay = a.No - 2, ax = a.No - 1;
+    // the highest axis whose fill wrote a constant wins (later fills overwrite)
+    if (cx <= kConstFront) return a.cst[ax][cx == kConstFront ? 0 : 1];
+    if (cy <= kConstFront) return a.cst[ay][cy == kConstFront ? 0 : 1];
+    if (oc_axis >= 0) return ocval;
+    if (cx == kUnwritten || cy == kUnwritten || ounw) return Arith<T>::zero();
+    return a.x[xoff + int64_t(cy) * a.xstr[ay] + int64_t(cx) * a.xstr[ax]];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) {
+    extern __shared__ unsigned char smem_raw[];
+    T* region = reinterpret_cast<T*>(smem_raw);
+    T* taps = region + a.RY * a.RX;
+
+    const int ay = a.No - 2, ax = a.No - 1;
+    int64_t bid = blockIdx.x;
+    const int tx = int(bid % a.ntx);
+    bid /= a.ntx;
+    const int ty = int(bid % a.nty);
+    int64_t outer = bid / a.nty;
+
+    // outer output coordinates (axes 0..No-3)
+    int64_t oo[kMaxRank];
+    int64_t yoff = 0;
+    {
+        int64_t rem = outer;
+        int64_t ostr = a.out[ay] * a.out[ax];
+        for (int e = a.No - 3; e >= 0; --e) {
+            oo[e] = rem % a.out[e];
+            rem /= a.out[e];
+            yoff += oo[e] * ostr;
+            ostr *= a.out[e];
+        }
+    }
+    const int64_t oy0 = int64_t(ty) * kTileY, ox0 = int64_t(tx) * kTileX;
+    const int lx = threadIdx.x % kTileX;
+    const int ly0 = threadIdx.x / kTileX;
+
+    T acc[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::zero();
+
+    const int64_t sy = a.s[ay], sx = a.s[ax], dy = a.d[ay], dx = a.d[ax];
+    // region offsets: gcd(s, d) divides both terms, so the subsampled index is
+    // ly*(s/g) + cy*(d/g)
+    const int sy_ = int(sy / a.gy), dy_ = int(dy / a.gy), sx_ = int(sx / a.gx), dx_ = int(dx / a.gx);
+    int base[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) base[i] = (ly0 + i * (kThreads / kTileX)) * sy_ * a.RX + lx * sx_;
+
+    for (int64_t to = 0; to < a.ktaps_outer; ++to) {
+        // outer tap -> padded coordinate per outer axis -> border code
+        int oc_axis = -1;
+        T ocval = Arith<T>::zero();
+        bool ounw = false;
+        int64_t xoff = 0, koff = 0;
+        {
+            int64_t rem = to;
+            for (int e = a.No - 3; e >= 0; --e) {
+                const int64_t t = rem % a.K[e];
+                rem /= a.K[e];
+                const int64_t pe = oo[e] * a.s[e] + t * a.d[e];
+                const int32_t c = a.map[e][pe];
+                if (c >= 0) xoff += int64_t(c) * a.xstr[e];
+                else if (c == kUnwritten) ounw = true;
+                else if (oc_axis < e) { oc_axis = e; ocval = a.cst[e][c == kConstFront ? 0 : 1]; }
+                koff += (a.rev ? a.K[e] - 1 - t : t) * a.kstr[e];
+            }
+        }
+        for (int ky0 = 0; ky0 < a.K[ay]; ky0 += a.KYc) {
+            const int kyn = int(a.K[ay] - ky0 < a.KYc ? a.K[ay] - ky0 : a.KYc);
+            for (int kx0 = 0; kx0 < a.K[ax]; kx0 += a.KXc) {
+                const int kxn = int(a.K[ax] - kx0 < a.KXc ? a.K[ax] - kx0 : a.KXc);
+                __syncthreads();
+                // stage the input region: rows py = oy0*sy + ky0*dy + r*gy
+                const int64_t py0 = oy0 * sy + int64_t(ky0) * dy;
+                const int64_t px0 = ox0 * sx + int64_t(kx0) * dx;
+                const int64_t leny = (a.out[ay] - 1) * sy + (a.K[ay] - 1) * dy + 1;  // padded extent used
+                const int64_t lenx = (a.out[ax] - 1) * sx + (a.K[ax] - 1) * dx + 1;
+                const int nreg = a.RY * a.RX;
+                for (int i = threadIdx.x; i < nreg; i += kThreads) {
+                    const int r = i / a.RX, c = i - r * a.RX;
+                    const int64_t py = py0 + int64_t(r) * a.gy;
+                    const int64_t px = px0 + int64_t(c) * a.gx;
+                    T v = Arith<T>::zero();
+                    if (py < leny && px < lenx) {
+                        const int32_t cy = a.map[ay][py];
+                        const int32_t cx = a.map[ax][px];
+                        v = resolve_plane(a, cy, cx, oc_axis, ocval, ounw, xoff);
+                    }
+                    region[i] = v;
+                }
+                // stage the kernel taps of this chunk (value of reversed-view tap t)
+                for (int i = threadIdx.x; i < kyn * kxn; i += kThreads) {
+                    const int cy = i / kxn, cx = i - cy * kxn;
+                    const int64_t ty_ = ky0 + cy, tx_ = kx0 + cx;
+                    const int64_t kiy = a.rev ? a.K[ay] - 1 - ty_ : ty_;
+                    const int64_t kix = a.rev ? a.K[ax] - 1 - tx_ : tx_;
+                    taps[i] = a.k[koff + kiy * a.kstr[ay] + kix * a.kstr[ax]];
+                }
+                __syncthreads();
+                for (int cy = 0; cy < kyn; ++cy) {
+                    const int roff = cy * dy_ * a.RX;
+                    for (int cx = 0; cx < kxn; ++cx) {
+                        const T v = taps[cy * kxn + cx];
+                        if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
+                        const int off = roff + cx * dx_;
+#pragma unroll
+                        for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::mac(acc[i], region[base[i] + off], v);
+                    }
+                }
+            }
+        }
+    }
+    const int64_t ox = ox0 + lx;
+    if (ox < a.out[ax]) {
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            const int64_t oy = oy0 + ly0 + i * (kThreads / kTileX);
+            if (oy < a.out[ay]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
+        }
+    }
+}
+
+int64_t gcd64(int64_t a, int64_t b) {
+    while (b) {
+        int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+template <typename T>
+ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx,
+                        T (*conv_const)(const ndc_value&)) {
+    DirectArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    int64_t offs[kMaxRank + 1];
+    const int32_t* maps = ctx.border_maps(p, offs);
+    if (!maps) return NDC_ERR_OUT_OF_MEMORY;
+    const int shift = p.N == 1 ? 1 : 0;   // promote rank 1 to a unit-row plane
+    a.No = p.N + shift;
+    if (shift) {
+        a.map[0] = maps + offs[kMaxRank];
+        a.xstr[0] = 0;
+        a.kstr[0] = 0;
+        a.s[0] = 1;
+        a.d[0] = 1;
+        a.out[0] = 1;
+        a.K[0] = 1;
+        a.cst[0][0] = a.cst[0][1] = conv_const(ndc_value{0, 0, 0});
+    }
+    for (int i = 0; i < p.N; ++i) {
+        const int j = i + shift;
+        a.map[j] = maps + offs[i];
+        a.xstr[j] = p.x_str[i];
+        a.kstr[j] = p.k_str[i];
+        a.s[j] = p.stride[i];
+        a.d[j] = p.dil[i];
+        a.out[j] = p.out[i];
+        a.K[j] = p.k_shape[i];
+        a.cst[j][0] = conv_const(p.side[i][0].value);
+        a.cst[j][1] = conv_const(p.side[i][1].value);
+    }
+    a.x = static_cast<const T*>(x);
+    a.k = static_cast<const T*>(k);
+    a.y = static_cast<T*>(y);
+    a.rev = p.reverse ? 1 : 0;
+    const int ay = a.No - 2, ax = a.No - 1;
+    a.nouter = 1;
+    a.ktaps_outer = 1;
+    for (int e = 0; e < a.No - 2; ++e) {
+        a.nouter *= a.out[e];
+        a.ktaps_outer *= a.K[e];
+    }
+    a.gy = int(gcd64(a.s[ay], a.d[ay]));
+    a.gx = int(gcd64(a.s[ax], a.d[ax]));
+    if (a.gy <= 0) a.gy = 1;
+    if (a.gx <= 0) a.gx = 1;
+    // Choose the kernel chunk so that region + taps fit the LDS budget.
+    const size_t budget = 48 * 1024;
+    auto region_elems = [&](int64_t kyc, int64_t kxc) {
+        const int64_t ry = ((kTileY - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
+        const int64_t rx = ((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1;
+        return std::make_pair(ry, rx);
+    };
+    int64_t kyc = a.K[ay], kxc = a.K[ax];
+    for (;;) {
+        auto rr = region_elems(kyc, kxc);
+        const size_t bytes = size_t(rr.first * rr.second + kyc * kxc) * sizeof(T);
+        if (bytes <= budget || (kyc == 1 && kxc == 1)) break;
+        if (kyc >= kxc && kyc > 1) kyc = (kyc + 1) / 2;
+        else if (kxc > 1) kxc = (kxc + 1) / 2;
+        else kyc = (kyc + 1) / 2;
+    }
+    auto rr = region_elems(kyc, kxc);
+    const size_t smem = size_t(rr.first * rr.second + kyc * kxc) * sizeof(T);
+    if (smem > ctx.max_lds()) {
+        ctx.err = "direct convolution: stride/dilation too large for the LDS tile";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    a.KYc = int(kyc);
+    a.KXc = int(kxc);
+    a.RY = int(rr.first);
+    a.RX = int(rr.second);
+    a.ntx = int((a.out[ax] + kTileX - 1) / kTileX);
+    a.nty = int((a.out[ay] + kTileY - 1) / kTileY);
+    const int64_t nblocks = a.nouter * a.ntx * a.nty;
+    if (nblocks <= 0) return NDC_OK;
+    if (nblocks > (int64_t(1) << 31) - 1) {
+        ctx.err = "direct convolution: output too large for one launch";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    ctx.kbegin("direct_conv", double(sizeof(T)) * double(p.x_numel + p.k_numel + p.y_numel),
+               2.0 * double(p.y_numel) * double(p.k_numel));
+    if (!lds_fits(ctx, reinterpret_cast<const void*>(&direct_conv_kernel<T>), smem)) return NDC_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(direct_conv_kernel<T>, dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(), a);
+    ctx.kend();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx.err = std::string("direct convolution launch: ") + hipGetErrorString(e);
+        return NDC_ERR_DEVICE;
+    }
+    return NDC_OK;
+}
+
+template <typename T> T cst_real(const ndc_value& v) { return T(v.re); }
+template <typename T> T cst_int(const ndc_value& v) { return T(v.i); }
+float2 cst_c64(const ndc_value& v) { return cmake<float2>(float(v.re), float(v.im)); }
+double2 cst_c128(const ndc_value& v) { return cmake<double2>(v.re, v.im); }
+
+}  // namespace
+
+ndc_status launch_direct(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx) {
+    switch (p.dtype) {
+        case NDC_F32: return launch_typed<float>(p, x, k, y, ctx, cst_real<float>);
+        case NDC_F64: return launch_typed<double>(p, x, k, y, ctx, cst_real<double>);
+        case NDC_C64: return launch_typed<float2>(p, x, k, y, ctx, cst_c64);
+        case NDC_C128: return launch_typed<double2>(p, x, k, y, ctx, cst_c128);
+        case NDC_I8: return launch_typed<int8_t>(p, x, k, y, ctx, cst_int<int8_t>);
+        case NDC_I16: return launch_typed<int16_t>(p, x, k, y, ctx, cst_int<int16_t>);
+        case NDC_I32: return launch_typed<int32_t>(p, x, k, y, ctx, cst_int<int32_t>);
+        case NDC_I64: return launch_typed<int64_t>(p, x, k, y, ctx, cst_int<int64_t>);
+        case NDC_U8: return launch_typed<uint8_t>(p, x, k, y, ctx, cst_int<uint8_t>);
+        case NDC_U16: return launch_typed<uint16_t>(p, x, k, y, ctx, cst_int<uint16_t>);
+        case NDC_U32: return launch_typed<uint32_t>(p, x, k, y, ctx, cst_int<uint32_t>);
+        case NDC_U64: return launch_typed<uint64_t>(p, x, k, y, ctx, cst_int<uint64_t>);
+        default: return NDC_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace ndc

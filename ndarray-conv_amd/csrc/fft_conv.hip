@@ -1,0 +1,1408 @@
+// FFT convolution on gfx950 (conv_fft/mod.rs:185-292).
+//
+// The padded data and the scattered kernel are never materialised: the first
+// pass reads x through the per-axis border maps (conv_fft/padding.rs:30-62,
+// padding/half_dim.rs) and k through the dilation/reversal scatter rule
+// (conv_fft/padding.rs:78-111) while loading rows into LDS.  Transforms are
+// mixed-radix Stockham FFTs over lines held in LDS, with FP64-generated
+// twiddles.  Axis order and layout:
+//
+//   pass 1  rows (last axis): r2c by the half-length trick (even F) or a full
+//           complex FFT (odd F / complex input), data and kernel in one grid;
+//   pass 2  axes N-2 .. 1 forward, columns in place, data and kernel together;
+//   pass 3  axis 0: forward data and kernel columns, multiply, inverse, keep
+//           only the rows the crop needs (conv_fft/mod.rs:268,278 and :282-289);
+//   pass 4  axes 1 .. N-2 inverse, cropped rows only;
+//   pass 5  rows inverse (c2r), scale 1/prod F (real.rs:278-279,
+//           complex.rs:141-142), crop and stride on store.
+// A rank-1 problem runs as one fused kernel.  Rows beyond the padded extent
+// are known to be zero and are neither read nor written.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace ndc {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxStages = 24;
+
+__device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+template <typename R>
+struct LinePlan {
+    int L;                 // complex length transformed
+    int nst;               // number of Stockham stages
+    int tws;               // twiddle table holds T = L * tws roots: W_T^i
+    int T;
+    const Cx<R>* tw;
+    int rad[kMaxStages];
+};
+
+// ---------------------------------------------------------------------------
+// Small DFT butterflies.  Forward uses W = exp(-2*pi*i/P); inverse the
+// conjugate, unnormalised (rustfft's convention).
+// ---------------------------------------------------------------------------
+
+template <typename C>
+__device__ __forceinline__ void bfly2(C* v) {
+    C a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+}
+
+template <typename C>
+__device__ __forceinline__ void bfly4(C* v, bool inv) {
+    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    C r = inv ? cmul_i(d13) : cmul_negi(d13);
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, r);
+    v[3] = csub(d02, r);
+}
+
+template <typename C>
+__device__ __forceinline__ void bfly8(C* v, bool inv) {
+    using R = decltype(C{}.x);
+    const R h = R(0.70710678118654757);
+    C a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        a[r] = cadd(v[r], v[r + 4]);
+        b[r] = csub(v[r], v[r + 4]);
+    }
+    // b[r] *= W_8^r
+    if (!inv) {
+        b[1] = cmake<C>(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+        b[2] = cmul_negi(b[2]);
+        b[3] = cmake<C>(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+    } else {
+        b[1] = cmake<C>(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
+        b[2] = cmul_i(b[2]);
+        b[3] = cmake<C>(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
+    }
+    bfly4(a, inv);
+    bfly4(b, inv);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[2 * q] = a[q];
+        v[2 * q + 1] = b[q];
+    }
+}
+
+// cos/sin(2*pi*m/P), m = 1 .. (P-1)/2
+template <int P> struct OddConst;
+template <> struct OddConst<3> {
+    static constexpr double c[2] = {1.0, -0.5};
+    static constexpr double s[2] = {0.0, 0.86602540378443865};
+};
+template <> struct OddConst<5> {
+    static constexpr double c[3] = {1.0, 0.30901699437494745, -0.80901699437494745};
+    static constexpr double s[3] = {0.0, 0.95105651629515353, 0.58778525229247314};
+};
+template <> struct OddConst<7> {
+    static constexpr double c[4] = {1.0, 0.62348980185873359, -0.22252093395631434, -0.90096886790241915};
+    static constexpr double s[4] = {0.0, 0.78183148246802980, 0.97492791218182362, 0.43388373911755812};
+};
+
+// y[k] = v0 + sum_n a_n cos(th n k) -/+ i sum_n b_n sin(th n k), with
+// a_n = v[n] + v[P-n], b_n = v[n] - v[P-n].
+template <int P, typename C>
+__device__ __forceinline__ void bfly_odd(C* v, bool inv) {
+    using R = decltype(C{}.x);
+    constexpr int H = (P - 1) / 2;
+    C a[H + 1], b[H + 1];
+#pragma unroll
+    for (int n = 1; n <= H; ++n) {
+        a[n] = cadd(v[n], v[P - n]);
+        b[n] = csub(v[n], v[P - n]);
+    }
+    C y0 = v[0];
+#pragma unroll
+    for (int n = 1; n <= H; ++n) y0 = cadd(y0, a[n]);
+    C out[P];
+    out[0] = y0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        C A = v[0], B = cmake<C>(R(0), R(0));
+#pragma unroll
+        for (int n = 1; n <= H; ++n) {
+            const int m = (n * k) % P;
+            const R cs = R(OddConst<P>::c[m <= H ? m : P - m]);
+            const R sn = m <= H ? R(OddConst<P>::s[m]) : R(-OddConst<P>::s[P - m]);
+            A = cadd(A, cscale(a[n], cs));
+            B = cadd(B, cscale(b[n], sn));
+        }
+        // forward: -i*B ; inverse: +i*B
+        const C t = inv ? cmul_i(B) : cmul_negi(B);
+        out[k] = cadd(A, t);
+        out[P - k] = csub(A, t);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = out[k];
+}
+
+template <int P, typename C>
+__device__ __forceinline__ void bfly(C* v, bool inv) {
+    if constexpr (P == 2) bfly2(v);
+    else if constexpr (P == 4) bfly4(v, inv);
+    else if constexpr (P == 8) bfly8(v, inv);
+    else bfly_odd<P>(v, inv);
+}
+
+// ---------------------------------------------------------------------------
+// Stockham stages over lines in LDS: element (line b, index t) at b*ls + t*es.
+// LINE_FAST selects the thread->item order: line index fastest for column
+// tiles (es = #lines, consecutive lines adjacent), element index fastest for
+// rows (es = 1).
+// ---------------------------------------------------------------------------
+
+template <typename R, int P, bool LINE_FAST>
+__device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
+                                            int nlines, int ls, int es, const LinePlan<R>& pl, bool inv) {
+    using C = Cx<R>;
+    const int nbf = L / P;
+    const int total = nbf * nlines;
+    const int twstep = pl.tws * (L / (Ns * P));
+    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+        int b, j;
+        if (LINE_FAST) { j = it / nlines; b = it - j * nlines; }
+        else { b = it / nbf; j = it - b * nbf; }
+        const int k = j % Ns;
+        const C* s = src + b * ls;
+        C v[P];
+#pragma unroll
+        for (int r = 0; r < P; ++r) v[r] = s[(j + r * nbf) * es];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < P; ++r) v[r] = cmulc(v[r], pl.tw[r * k * twstep], inv);
+        }
+        bfly<P>(v, inv);
+        C* d = dst + b * ls;
+        const int o = (j - k) * P + k;
+#pragma unroll
+        for (int r = 0; r < P; ++r) d[(o + r * Ns) * es] = v[r];
+    }
+}
+
+// Any radix: one output per item, O(P) reads (used for the primes above 7
+// that good_size can produce: 11, 13, 17, 31, 43, 73, 127, 257).
+template <typename R, bool LINE_FAST>
+__device__ __forceinline__ void stage_generic(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
+                                              int P, int nlines, int ls, int es, const LinePlan<R>& pl, bool inv) {
+    using C = Cx<R>;
+    const int nbf = L / P;
+    const int total = L * nlines;
+    const int twstep = pl.tws * (L / (Ns * P));
+    const int wp = pl.T / P;
+    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+        int b, o;
+        if (LINE_FAST) { o = it / nlines; b = it - o * nlines; }
+        else { b = it / L; o = it - b * L; }
+        const int k = o % Ns;
+        const int t = o / Ns;
+        const int rout = t % P;
+        const int j = (t / P) * Ns + k;
+        const C* s = src + b * ls;
+        C acc = cmake<C>(R(0), R(0));
+        int m = 0;   // (r * rout) mod P
+        for (int r = 0; r < P; ++r) {
+            C v = s[(j + r * nbf) * es];
+            if (Ns > 1 && r) v = cmulc(v, pl.tw[r * k * twstep], inv);
+            acc = cadd(acc, cmulc(v, pl.tw[m * wp], inv));
+            m += rout;
+            if (m >= P) m -= P;
+        }
+        dst[b * ls + o * es] = acc;
+    }
+}
+
+// Runs all stages; input in buf[0]; returns the index (0/1) of the buffer
+// holding the output.  Ends with a barrier.
+template <typename R, bool LINE_FAST>
+__device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, int nlines, int ls, int es, bool inv) {
+    Cx<R>* bufs[2] = {b0, b1};
+    int cur = 0;
+    int Ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int P = pl.rad[s];
+        const Cx<R>* src = bufs[cur];
+        Cx<R>* dst = bufs[cur ^ 1];
+        switch (P) {
+            case 2: stage_fixed<R, 2, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            case 3: stage_fixed<R, 3, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            case 4: stage_fixed<R, 4, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            case 5: stage_fixed<R, 5, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            case 7: stage_fixed<R, 7, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            case 8: stage_fixed<R, 8, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
+            default: stage_generic<R, LINE_FAST>(src, dst, pl.L, Ns, P, nlines, ls, es, pl, inv); break;
+        }
+        __syncthreads();
+        cur ^= 1;
+        Ns *= P;
+    }
+    return cur;
+}
+
+// ---------------------------------------------------------------------------
+// Load stages
+// ---------------------------------------------------------------------------
+
+// What the data side needs to read the padded input through the border maps.
+template <typename E>
+struct PadSrc {
+    const E* x;
+    const int32_t* map[kMaxRank];   // length fft[e]
+    E cst[kMaxRank][2];
+    int64_t xstr[kMaxRank];
+};
+
+// What the kernel side needs to read the scattered, dilated kernel.
+template <typename E>
+struct KerSrc {
+    const E* k;
+    int64_t kstr[kMaxRank];
+    int32_t K[kMaxRank];
+    int32_t d[kMaxRank];
+    int32_t rev;
+};
+
+// Outer-coordinate resolution of one data row.
+template <typename E>
+struct RowRes {
+    int64_t xoff;
+    int oc_axis;    // highest outer axis whose fill wrote a constant, or -1
+    E ocval;
+    bool unw;       // some outer axis is an unwritten pad cell
+    bool zero;      // some outer coordinate lies beyond the padded extent
+};
+
+template <typename E>
+__device__ __forceinline__ E zero_of() {
+    if constexpr (std::is_same<E, float2>::value || std::is_same<E, double2>::value) {
+        E z;
+        z.x = 0;
+        z.y = 0;
+        return z;
+    } else {
+        return E(0);
+    }
+}
+
+template <typename E>
+__device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, int ax, int32_t c) {
+    if (rr.zero || c == kOutside) return zero_of<E>();
+    if (c <= kConstFront) return s.cst[ax][c == kConstFront ? 0 : 1];
+    if (rr.oc_axis >= 0) return rr.ocval;
+    if (c == kUnwritten || rr.unw) return zero_of<E>();
+    return s.x[rr.xoff + int64_t(c) * s.xstr[ax]];
+}
+
+template <typename R, typename E>
+__device__ __forceinline__ Cx<R> to_cx(E v) {
+    if constexpr (std::is_same<E, Cx<R>>::value) return v;
+    else return cmake<Cx<R>>(R(v), R(0));
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: forward rows (last axis) for data (job 0) and kernel (job 1)
+// ---------------------------------------------------------------------------
+
+template <typename R, typename E>
+struct RowsFwdArgs {
+    PadSrc<E> pad;
+    KerSrc<E> ker;
+    Cx<R>* spec[2];          // output spectra (data, kernel)
+    int64_t sstr[kMaxRank];  // spectrum strides of the outer axes (complex elements)
+    int64_t cnt[2][kMaxRank];// active extent of each outer axis per job
+    int64_t nlines[2];
+    int32_t nblocks0;        // blocks of job 0
+    int32_t nout;            // number of outer axes (N-1)
+    int32_t F;               // FFT size of the last axis
+    int32_t H;               // bins written per row
+    int32_t mode;            // 0: real even (half-length), 1: real odd (full complex), 2: complex input
+    int32_t lpb;             // lines per block
+    int32_t ls;              // LDS line stride (complex elements)
+    LinePlan<R> pl;
+};
+
+template <typename R, typename E>
+__global__ void __launch_bounds__(kThreads) rows_fwd_kernel(RowsFwdArgs<R, E> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + a.lpb * a.ls;
+    __shared__ int64_t s_off[16];
+    __shared__ int64_t s_xoff[16];
+    __shared__ int s_flag[16];     // bit0 zero row, bit1 unwritten, oc axis in bits 8..
+    __shared__ E s_oc[16];
+
+    const int job = blockIdx.x < unsigned(a.nblocks0) ? 0 : 1;
+    const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int64_t line0 = blk * a.lpb;
+    const int nl = int(imin64(a.lpb, a.nlines[job] - line0));
+    const int ax = a.nout;   // last axis index
+
+    // per-line outer resolution
+    if (threadIdx.x < nl) {
+        int64_t rem = line0 + threadIdx.x;
+        int64_t soff = 0, xoff = 0;
+        int flag = 0;
+        int oc = -1;
+        E ocv = zero_of<E>();
+        for (int e = a.nout - 1; e >= 0; --e) {
+            const int64_t q = rem % a.cnt[job][e];
+            rem /= a.cnt[job][e];
+            soff += q * a.sstr[e];
+            if (job == 0) {
+                const int32_t c = a.pad.map[e][q];
+                if (c >= 0) xoff += int64_t(c) * a.pad.xstr[e];
+                else if (c == kOutside) flag |= 1;
+                else if (c == kUnwritten) flag |= 2;
+                else if (oc < e) { oc = e; ocv = a.pad.cst[e][c == kConstFront ? 0 : 1]; }
+            } else {
+                if (q % a.ker.d[e] != 0) flag |= 1;
+                const int64_t i = q / a.ker.d[e];
+                if (i >= a.ker.K[e]) flag |= 1;
+                const int64_t ki = a.ker.rev ? i : a.ker.K[e] - 1 - i;
+                xoff += ki * a.ker.kstr[e];
+            }
+        }
+        s_off[threadIdx.x] = soff;
+        s_xoff[threadIdx.x] = xoff;
+        s_flag[threadIdx.x] = flag | ((oc + 1) << 8);
+        s_oc[threadIdx.x] = ocv;
+    }
+    __syncthreads();
+
+    // load
+    const int L = a.pl.L;
+    const int total = nl * L;
+    for (int it = threadIdx.x; it < total; it += kThreads) {
+        const int b = it / L, n = it - b * L;
+        const int flag = s_flag[b];
+        E v0 = zero_of<E>(), v1 = zero_of<E>();
+        if (!(flag & 1)) {
+            if (job == 0) {
+                RowRes<E> rr{s_xoff[b], (flag >> 8) - 1, s_oc[b], (flag & 2) != 0, false};
+                if (a.mode == 0) {
+                    v0 = pad_value(a.pad, rr, ax, a.pad.map[ax][2 * n]);
+                    v1 = pad_value(a.pad, rr, ax, a.pad.map[ax][2 * n + 1]);
+                } else {
+                    v0 = pad_value(a.pad, rr, ax, a.pad.map[ax][n]);
+                }
+            } else {
+                const int dd = a.ker.d[ax];
+                auto kv = [&](int p) -> E {
+                    if (p % dd) return zero_of<E>();
+                    const int i = p / dd;
+                    if (i >= a.ker.K[ax]) return zero_of<E>();
+                    const int ki = a.ker.rev ? i : a.ker.K[ax] - 1 - i;
+                    return a.ker.k[s_xoff[b] + int64_t(ki) * a.ker.kstr[ax]];
+                };
+                if (a.mode == 0) { v0 = kv(2 * n); v1 = kv(2 * n + 1); }
+                else v0 = kv(n);
+            }
+        }
+        C z;
+        if (a.mode == 0) {
+            if constexpr (std::is_same<E, C>::value) z = v0;   // not reached: complex rows use mode 2
+            else z = cmake<C>(R(v0), R(v1));
+        } else {
+            z = to_cx<R>(v0);
+        }
+        b0[b * a.ls + n] = z;
+    }
+    __syncthreads();
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, nl, a.ls, 1, false);
+    const C* Z = cur ? b1 : b0;
+
+    // store H bins per row
+    Cx<R>* out = a.spec[job];
+    const int H = a.H;
+    const int M = L;
+    for (int it = threadIdx.x; it < nl * H; it += kThreads) {
+        const int b = it / H, k = it - b * H;
+        C X;
+        if (a.mode == 0) {
+            // X[k] = E[k] + W_F^k O[k], E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2
+            const C zk = Z[b * a.ls + (k == M ? 0 : k)];
+            const C zm = cconj(Z[b * a.ls + (k == 0 ? 0 : M - k)]);
+            const C e = cscale(cadd(zk, zm), R(0.5));
+            const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
+            X = cadd(e, cmul(o, a.pl.tw[k]));
+        } else {
+            X = Z[b * a.ls + k];
+        }
+        out[s_off[b] + k] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Passes 2-4: column transforms along axis d of the spectrum (in place)
+// ---------------------------------------------------------------------------
+
+struct AxisSel {
+    int64_t start, step, count;
+};
+
+template <typename R>
+struct ColsArgs {
+    Cx<R>* spec[2];
+    int64_t sstr[kMaxRank];          // spectrum strides (complex elements)
+    AxisSel outer[2][kMaxRank];      // selection per outer axis per job (axes != d, < N-1)
+    int32_t outer_axis[kMaxRank];    // which spectrum axis each outer index is
+    int32_t nouter;
+    int64_t nlines_outer[2];         // product of outer counts per job
+    int32_t ntiles;                  // bin tiles along the last axis
+    int32_t nblocks0;
+    int32_t H;                       // bins
+    int32_t C;                       // tile width (lines per block)
+    int32_t d;                       // transformed axis
+    int32_t valid[2];                // rows of axis d holding data (rest are zero)
+    AxisSel outsel;                  // rows of axis d to write back
+    int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse
+    int32_t ls_pad;                  // unused padding
+    LinePlan<R> pl;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kThreads) cols_kernel(ColsArgs<R> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    const int L = a.pl.L;
+    const int Ct = a.C;
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + L * Ct;
+    C* b2 = b1 + L * Ct;
+
+    const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
+    int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int tile = int(blk % a.ntiles);
+    int64_t rem = blk / a.ntiles;
+    int64_t base = 0;
+    for (int i = a.nouter - 1; i >= 0; --i) {
+        const AxisSel& s = a.outer[job][i];
+        const int64_t q = rem % s.count;
+        rem /= s.count;
+        base += (s.start + q * s.step) * a.sstr[a.outer_axis[i]];
+    }
+    const int c0 = tile * Ct;
+    const int nc = min(Ct, a.H - c0);
+    const int64_t sd = a.sstr[a.d];
+
+    auto load = [&](const C* src, C* dstbuf, int valid) {
+        const int total = L * Ct;
+        for (int it = threadIdx.x; it < total; it += kThreads) {
+            const int t = it / Ct, c = it - t * Ct;
+            C v = cmake<C>(R(0), R(0));
+            if (t < valid && c < nc) v = src[base + int64_t(t) * sd + c0 + c];
+            dstbuf[t * Ct + c] = v;
+        }
+    };
+
+    C* res;
+    if (a.mode == 0) {
+        load(a.spec[job], b0, a.valid[job]);
+        __syncthreads();
+        res = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, false) ? b1 : b0;
+    } else if (a.mode == 1) {
+        // kernel column -> forward; result parked in b2 (or already there)
+        load(a.spec[1], b1, a.valid[1]);
+        __syncthreads();
+        const int ck = fft_lines<R, true>(b1, b2, a.pl, Ct, 1, Ct, false);
+        C* kres = ck ? b2 : b1;
+        C* kfree = ck ? b1 : b2;
+        // data column -> forward using b0 and the free buffer
+        load(a.spec[0], b0, a.valid[0]);
+        __syncthreads();
+        const int cd = fft_lines<R, true>(b0, kfree, a.pl, Ct, 1, Ct, false);
+        C* dres = cd ? kfree : b0;
+        C* dfree = cd ? b0 : kfree;
+        for (int it = threadIdx.x; it < L * Ct; it += kThreads) dres[it] = cmul(dres[it], kres[it]);
+        __syncthreads();
+        res = fft_lines<R, true>(dres, dfree, a.pl, Ct, 1, Ct, true) ? dfree : dres;
+    } else {
+        load(a.spec[0], b0, L);
+        __syncthreads();
+        res = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, true) ? b1 : b0;
+    }
+    // store selected rows
+    C* dst = a.spec[job];
+    const int nrow = int(a.outsel.count);
+    for (int it = threadIdx.x; it < nrow * Ct; it += kThreads) {
+        const int i = it / Ct, c = it - i * Ct;
+        if (c >= nc) continue;
+        const int t = int(a.outsel.start + int64_t(i) * a.outsel.step);
+        dst[base + int64_t(t) * sd + c0 + c] = res[t * Ct + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 5: inverse rows (last axis), scale, crop, stride
+// ---------------------------------------------------------------------------
+
+template <typename R, typename E>
+struct RowsInvArgs {
+    const Cx<R>* spec;
+    E* y;
+    int64_t sstr[kMaxRank];
+    AxisSel sel[kMaxRank];        // crop selection per outer axis
+    int64_t ystr[kMaxRank];       // output strides of the outer axes
+    int64_t nlines;
+    int32_t nout;
+    int32_t F, H, mode, lpb, ls;
+    AxisSel lastsel;              // crop along the last axis
+    R scale;
+    LinePlan<R> pl;
+};
+
+template <typename R, typename E>
+__global__ void __launch_bounds__(kThreads) rows_inv_kernel(RowsInvArgs<R, E> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + a.lpb * a.ls;
+    __shared__ int64_t s_off[16];
+    __shared__ int64_t s_yoff[16];
+    const int64_t line0 = int64_t(blockIdx.x) * a.lpb;
+    const int nl = int(imin64(a.lpb, a.nlines - line0));
+    if (threadIdx.x < nl) {
+        int64_t rem = line0 + threadIdx.x;
+        int64_t soff = 0, yoff = 0;
+        for (int e = a.nout - 1; e >= 0; --e) {
+            const int64_t q = rem % a.sel[e].count;
+            rem /= a.sel[e].count;
+            soff += (a.sel[e].start + q * a.sel[e].step) * a.sstr[e];
+            yoff += q * a.ystr[e];
+        }
+        s_off[threadIdx.x] = soff;
+        s_yoff[threadIdx.x] = yoff;
+    }
+    __syncthreads();
+    const int L = a.pl.L;
+    const int M = L;
+    for (int it = threadIdx.x; it < nl * L; it += kThreads) {
+        const int b = it / L, k = it - b * L;
+        const C* X = a.spec + s_off[b];
+        C z;
+        if (a.mode == 0) {
+            // Z[k] = (A + B) + i W_F^-k (A - B), A = X[k], B = conj X[M-k];
+            // Im X[0] and Im X[M] are ignored as a c2r transform does
+            C A = X[k], B = cconj(X[M - k]);
+            if (k == 0) { A.y = 0; B = cmake<C>(X[M].x, R(0)); }
+            const C s = cadd(A, B);
+            const C t = cmul(csub(A, B), cconj(a.pl.tw[k]));
+            z = cadd(s, cmul_i(t));
+        } else if (a.mode == 1) {
+            // odd F: Hermitian completion of the half spectrum
+            if (k == 0) z = cmake<C>(X[0].x, R(0));
+            else if (k < a.H) z = X[k];
+            else z = cconj(X[L - k]);
+        } else {
+            z = X[k];
+        }
+        b0[b * a.ls + k] = z;
+    }
+    __syncthreads();
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, nl, a.ls, 1, true);
+    const C* Z = cur ? b1 : b0;
+    const int nout_last = int(a.lastsel.count);
+    for (int it = threadIdx.x; it < nl * nout_last; it += kThreads) {
+        const int b = it / nout_last, i = it - b * nout_last;
+        const int p = int(a.lastsel.start + int64_t(i) * a.lastsel.step);
+        E v;
+        if constexpr (std::is_same<E, C>::value) {
+            v = cscale(Z[b * a.ls + p], a.scale);
+        } else if (a.mode == 0) {
+            const C zz = Z[b * a.ls + (p >> 1)];
+            v = E(((p & 1) ? zz.y : zz.x) * a.scale);
+        } else {
+            v = E(Z[b * a.ls + p].x * a.scale);
+        }
+        a.y[s_yoff[b] + i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rank 1: kernel and data transforms, multiply, inverse and crop in one block
+// ---------------------------------------------------------------------------
+
+template <typename R, typename E>
+struct Fused1dArgs {
+    PadSrc<E> pad;
+    KerSrc<E> ker;
+    const Cx<R>* ks;   // kernel spectrum bins computed by a previous pass, or null
+    E* y;
+    int32_t F, H, mode;
+    AxisSel lastsel;
+    R scale;
+    LinePlan<R> pl;
+};
+
+template <typename R, typename E>
+__global__ void __launch_bounds__(kThreads) fused1d_kernel(Fused1dArgs<R, E> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    const int L = a.pl.L;
+    C* buf[3] = {reinterpret_cast<C*>(smem_raw), reinterpret_cast<C*>(smem_raw) + L,
+                 reinterpret_cast<C*>(smem_raw) + 2 * L};
+    const RowRes<E> rr{0, -1, zero_of<E>(), false, false};
+    auto kv = [&](int p) -> E {
+        const int dd = a.ker.d[0];
+        if (p % dd) return zero_of<E>();
+        const int i = p / dd;
+        if (i >= a.ker.K[0]) return zero_of<E>();
+        const int ki = a.ker.rev ? i : a.ker.K[0] - 1 - i;
+        return a.ker.k[int64_t(ki) * a.ker.kstr[0]];
+    };
+    auto load = [&](C* dst, bool kernel) {
+        for (int n = threadIdx.x; n < L; n += kThreads) {
+            C z;
+            if (a.mode == 0) {
+                E v0, v1;
+                if (kernel) { v0 = kv(2 * n); v1 = kv(2 * n + 1); }
+                else { v0 = pad_value(a.pad, rr, 0, a.pad.map[0][2 * n]); v1 = pad_value(a.pad, rr, 0, a.pad.map[0][2 * n + 1]); }
+                if constexpr (std::is_same<E, C>::value) z = v0;
+                else z = cmake<C>(R(v0), R(v1));
+            } else {
+                E v0 = kernel ? kv(n) : pad_value(a.pad, rr, 0, a.pad.map[0][n]);
+                z = to_cx<R>(v0);
+            }
+            dst[n] = z;
+        }
+    };
+    // kernel spectrum: transformed here (three line buffers) or read from
+    // global memory (two line buffers)
+    C* K = nullptr;
+    C* f0 = buf[0];
+    C* f1 = buf[1];
+    if (!a.ks) {
+        load(buf[0], true);
+        __syncthreads();
+        const int ck = fft_lines<R, false>(buf[0], buf[1], a.pl, 1, L, 1, false);
+        K = buf[ck];
+        f0 = buf[ck ^ 1];
+        f1 = buf[2];
+    }
+    load(f0, false);
+    __syncthreads();
+    const int cd = fft_lines<R, false>(f0, f1, a.pl, 1, L, 1, false);
+    C* D = cd ? f1 : f0;
+    C* W = cd ? f0 : f1;   // free buffer
+    if (a.mode == 0) {
+        // bins k and M-k together: forward post-process both spectra,
+        // multiply, then the c2r pre-process of the product
+        const int M = L;
+        for (int k = threadIdx.x; k <= M / 2; k += kThreads) {
+            const int km = M - k;
+            auto bin = [&](const C* Z, int kk) {
+                const C zk = Z[kk == M ? 0 : kk];
+                const C zm = cconj(Z[kk == 0 ? 0 : M - kk]);
+                const C e = cscale(cadd(zk, zm), R(0.5));
+                const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
+                return cadd(e, cmul(o, a.pl.tw[kk]));
+            };
+            C Pk = cmul(bin(D, k), a.ks ? a.ks[k] : bin(K, k));
+            C Pm = cmul(bin(D, km), a.ks ? a.ks[km] : bin(K, km));
+            if (k == 0) { Pk.y = 0; Pm.y = 0; }   // Im of DC and Nyquist ignored by c2r
+            {
+                const C A = Pk, B = cconj(Pm);
+                const C t = cmul(csub(A, B), cconj(a.pl.tw[k]));
+                W[k == M ? 0 : k] = cadd(cadd(A, B), cmul_i(t));
+            }
+            if (km != k && km < M) {
+                const C A = Pm, B = cconj(Pk);
+                const C t = cmul(csub(A, B), cconj(a.pl.tw[km]));
+                W[km] = cadd(cadd(A, B), cmul_i(t));
+            }
+        }
+    } else if (a.mode == 1) {
+        // odd F: the product of two Hermitian spectra, used whole
+        for (int k = threadIdx.x; k < L; k += kThreads) {
+            // the stored kernel spectrum holds bins 0..F/2; the rest by symmetry
+            const C kk = a.ks ? (k < a.H ? a.ks[k] : cconj(a.ks[L - k])) : K[k];
+            W[k] = cmul(D[k], kk);
+        }
+        if (threadIdx.x == 0) W[0].y = 0;
+    } else {
+        for (int k = threadIdx.x; k < L; k += kThreads) W[k] = cmul(D[k], a.ks ? a.ks[k] : K[k]);
+    }
+    __syncthreads();
+    C* other = (W == f0) ? f1 : f0;
+    const int ci = fft_lines<R, false>(W, other, a.pl, 1, L, 1, true);
+    const C* Z = ci ? other : W;
+    const int nout = int(a.lastsel.count);
+    for (int i = threadIdx.x; i < nout; i += kThreads) {
+        const int p = int(a.lastsel.start + int64_t(i) * a.lastsel.step);
+        E v;
+        if constexpr (std::is_same<E, C>::value) {
+            v = cscale(Z[p], a.scale);
+        } else if (a.mode == 0) {
+            const C zz = Z[p >> 1];
+            v = E(((p & 1) ? zz.y : zz.x) * a.scale);
+        } else {
+            v = E(Z[p].x * a.scale);
+        }
+        a.y[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lines longer than LDS: four-step decomposition of the last axis.
+// A line of L = L1*L2 points is viewed as [n1][n2] (n = n1*L2 + n2):
+//   forward  FFT along n1 (stride L2), times W_L^(k1*n2), then FFT along n2,
+//            leaving X[k1 + L1*k2] at [k1][k2];
+//   inverse  FFT along k2, times conj W_L^(k1*n2), FFT along k1 -> natural
+//            order.
+// The spectrum stays in [k1][k2] order, which the elementwise product and
+// the other axes' passes do not care about.  Real input is transformed as
+// complex (all L bins kept).
+// ---------------------------------------------------------------------------
+
+template <typename R, typename E>
+struct BigColArgs {
+    PadSrc<E> pad;
+    KerSrc<E> ker;
+    Cx<R>* spec[2];
+    E* y;
+    int64_t sstr[kMaxRank];       // spectrum strides of the outer axes
+    AxisSel sel[2][kMaxRank];     // outer line selection per job
+    int64_t ystr[kMaxRank];       // output strides of the outer axes (inverse)
+    int64_t nlines[2];
+    int32_t nout;
+    int32_t L1, L2, Ct, ntiles, nblocks0;
+    int32_t inv;                  // 0: forward from x / k (two jobs); 1: inverse to y (crop)
+    AxisSel lastsel;              // crop along the long axis (inverse)
+    R scale;
+    const Cx<R>* twL;             // W_L^i, i < L
+    LinePlan<R> pl;               // length-L1 plan
+};
+
+template <typename R, typename E>
+__global__ void __launch_bounds__(kThreads) bigcol_kernel(BigColArgs<R, E> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    const int L1 = a.pl.L, L2 = a.L2, Ct = a.Ct;
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + L1 * Ct;
+    const int job = (!a.inv && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
+    const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int tile = int(blk % a.ntiles);
+    int64_t line = blk / a.ntiles;
+    // outer coordinates of this line
+    int64_t soff = 0, xoff = 0, yoff = 0;
+    int oc = -1;
+    E ocv = zero_of<E>();
+    bool zero = false, unw = false;
+    {
+        int64_t rem = line;
+        for (int e = a.nout - 1; e >= 0; --e) {
+            const AxisSel& s = a.sel[job][e];
+            const int64_t i = rem % s.count;
+            rem /= s.count;
+            const int64_t q = s.start + i * s.step;
+            soff += q * a.sstr[e];
+            yoff += i * a.ystr[e];
+            if (a.inv) continue;
+            if (job == 0) {
+                const int32_t c = a.pad.map[e][q];
+                if (c >= 0) xoff += int64_t(c) * a.pad.xstr[e];
+                else if (c == kOutside) zero = true;
+                else if (c == kUnwritten) unw = true;
+                else if (oc < e) { oc = e; ocv = a.pad.cst[e][c == kConstFront ? 0 : 1]; }
+            } else {
+                if (q % a.ker.d[e] != 0) zero = true;
+                const int64_t ii = q / a.ker.d[e];
+                if (ii >= a.ker.K[e]) zero = true;
+                xoff += (a.ker.rev ? ii : a.ker.K[e] - 1 - ii) * a.ker.kstr[e];
+            }
+        }
+    }
+    const int c0 = tile * Ct;
+    const int nc = min(Ct, L2 - c0);
+    const int ax = a.nout;
+    for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
+        const int n1 = it / Ct, c = it - n1 * Ct;
+        C v = cmake<C>(R(0), R(0));
+        if (c < nc) {
+            const int n2 = c0 + c;
+            if (!a.inv) {
+                const int64_t n = int64_t(n1) * L2 + n2;
+                if (!zero) {
+                    E e;
+                    if (job == 0) {
+                        RowRes<E> rr{xoff, oc, ocv, unw, false};
+                        e = pad_value(a.pad, rr, ax, a.pad.map[ax][n]);
+                    } else {
+                        const int dd = a.ker.d[ax];
+                        e = zero_of<E>();
+                        if (n % dd == 0 && n / dd < a.ker.K[ax]) {
+                            const int64_t i = n / dd;
+                            e = a.ker.k[xoff + (a.ker.rev ? i : a.ker.K[ax] - 1 - i) * a.ker.kstr[ax]];
+                        }
+                    }
+                    v = to_cx<R>(e);
+                }
+            } else {
+                v = cmulc(a.spec[0][soff + int64_t(n1) * L2 + n2], a.twL[int64_t(n1) * n2], true);
+            }
+        }
+        b0[n1 * Ct + c] = v;
+    }
+    __syncthreads();
+    const int cur = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, a.inv != 0);
+    const C* Z = cur ? b1 : b0;
+    if (!a.inv) {
+        C* out = a.spec[job];
+        for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
+            const int k1 = it / Ct, c = it - k1 * Ct;
+            if (c >= nc) continue;
+            const int n2 = c0 + c;
+            out[soff + int64_t(k1) * L2 + n2] = cmul(Z[k1 * Ct + c], a.twL[int64_t(k1) * n2]);
+        }
+    } else {
+        for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
+            const int n1 = it / Ct, c = it - n1 * Ct;
+            if (c >= nc) continue;
+            const int64_t n = int64_t(n1) * L2 + c0 + c;
+            const int64_t rel = n - a.lastsel.start;
+            if (rel < 0 || rel % a.lastsel.step) continue;
+            const int64_t i = rel / a.lastsel.step;
+            if (i >= a.lastsel.count) continue;
+            E v;
+            if constexpr (std::is_same<E, C>::value) v = cscale(Z[n1 * Ct + c], a.scale);
+            else v = E(Z[n1 * Ct + c].x * a.scale);
+            a.y[yoff + i] = v;
+        }
+    }
+}
+
+template <typename R>
+struct BigRowArgs {
+    Cx<R>* spec[2];
+    int64_t sstr[kMaxRank];
+    AxisSel sel[2][kMaxRank];
+    int64_t nrows[2];             // outer lines * L1 per job
+    int32_t nout, L1, L2, nblocks0;
+    int32_t mode;                 // 0 forward (two jobs), 1 forward x kernel -> inverse (job 0), 2 inverse
+    LinePlan<R> pl;               // length-L2 plan
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kThreads) bigrow_kernel(BigRowArgs<R> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    const int L2 = a.L2;
+    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* b1 = b0 + L2;
+    C* b2 = b1 + L2;
+    const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
+    const int64_t row = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int64_t k1 = row % a.L1;
+    int64_t rem = row / a.L1;
+    int64_t soff = k1 * L2;
+    for (int e = a.nout - 1; e >= 0; --e) {
+        const AxisSel& s = a.sel[job][e];
+        const int64_t i = rem % s.count;
+        rem /= s.count;
+        soff += (s.start + i * s.step) * a.sstr[e];
+    }
+    C* res;
+    if (a.mode == 1) {
+        for (int n = threadIdx.x; n < L2; n += kThreads) b1[n] = a.spec[1][soff + n];
+        __syncthreads();
+        const int ck = fft_lines<R, false>(b1, b2, a.pl, 1, L2, 1, false);
+        C* K = ck ? b2 : b1;
+        C* f1 = ck ? b1 : b2;
+        for (int n = threadIdx.x; n < L2; n += kThreads) b0[n] = a.spec[0][soff + n];
+        __syncthreads();
+        const int cd = fft_lines<R, false>(b0, f1, a.pl, 1, L2, 1, false);
+        C* D = cd ? f1 : b0;
+        C* Dfree = cd ? b0 : f1;
+        for (int n = threadIdx.x; n < L2; n += kThreads) D[n] = cmul(D[n], K[n]);
+        __syncthreads();
+        res = fft_lines<R, false>(D, Dfree, a.pl, 1, L2, 1, true) ? Dfree : D;
+    } else {
+        for (int n = threadIdx.x; n < L2; n += kThreads) b0[n] = a.spec[job][soff + n];
+        __syncthreads();
+        res = fft_lines<R, false>(b0, b1, a.pl, 1, L2, 1, a.mode == 2) ? b1 : b0;
+    }
+    for (int n = threadIdx.x; n < L2; n += kThreads) a.spec[job][soff + n] = res[n];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+std::vector<int> factor_radices(int L) {
+    std::vector<int> r;
+    int n = L;
+    while (n % 8 == 0 && n > 8) { r.push_back(8); n /= 8; }
+    if (n % 8 == 0) { r.push_back(8); n /= 8; }
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    for (int p : {3, 5, 7}) while (n % p == 0) { r.push_back(p); n /= p; }
+    for (int p = 11; n > 1; p += 2) {
+        while (n % p == 0) { r.push_back(p); n /= p; }
+        if (int64_t(p) * p > n && n > 1) { r.push_back(n); n = 1; }
+    }
+    return r;
+}
+
+template <typename R>
+bool make_lineplan(LinePlan<R>* pl, int L, int tws, LaunchContext& ctx) {
+    std::memset(pl, 0, sizeof(*pl));
+    pl->L = L;
+    pl->tws = tws;
+    pl->T = L * tws;
+    auto r = factor_radices(L);
+    if (int(r.size()) > kMaxStages) return false;
+    pl->nst = int(r.size());
+    for (size_t i = 0; i < r.size(); ++i) pl->rad[i] = r[i];
+    pl->tw = static_cast<const Cx<R>*>(ctx.twiddles(pl->T, sizeof(R) == 8));
+    return pl->tw != nullptr;
+}
+
+template <typename E>
+E host_value(const ndc_value& v);
+template <> float host_value<float>(const ndc_value& v) { return float(v.re); }
+template <> double host_value<double>(const ndc_value& v) { return v.re; }
+template <> float2 host_value<float2>(const ndc_value& v) { return cmake<float2>(float(v.re), float(v.im)); }
+template <> double2 host_value<double2>(const ndc_value& v) { return cmake<double2>(v.re, v.im); }
+
+ndc_status check_launch(LaunchContext& ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx.err = std::string(what) + ": " + hipGetErrorString(e);
+        return NDC_ERR_DEVICE;
+    }
+    return NDC_OK;
+}
+
+template <typename R, typename E>
+ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, LaunchContext& ctx) {
+    using C = Cx<R>;
+    constexpr bool kComplex = std::is_same<E, C>::value;
+    const int N = p.N;
+    const int last = N - 1;
+    const int Fl = int(p.fft[last]);
+    const size_t es = sizeof(C);
+    const size_t lds = ctx.max_lds();
+    int mode;             // 0 real even, 1 real odd, 2 complex
+    int L, tws;
+    if (kComplex) { mode = 2; L = Fl; tws = 1; }
+    else if (Fl % 2 == 0) { mode = 0; L = Fl / 2; tws = 2; }
+    else { mode = 1; L = Fl; tws = 1; }
+    // Lines that do not fit two LDS buffers take the four-step path, as
+    // complex transforms of the full length (all Fl bins kept).
+    const bool big = size_t(2) * L * es > lds;
+    const int H = big ? Fl : (kComplex ? Fl : Fl / 2 + 1);
+    int L1 = 0, L2 = 0, Ctb = 0;
+    if (big) {
+        for (int d2 = Fl; d2 >= 2; --d2) {
+            if (Fl % d2 || size_t(3) * d2 * es > lds) continue;
+            const int d1 = Fl / d2;
+            int ct = 16;
+            while (ct > 1 && size_t(2) * d1 * ct * es > lds) ct /= 2;
+            if (size_t(2) * d1 * ct * es > lds) continue;
+            L1 = d1;
+            L2 = d2;
+            Ctb = ct;
+            break;
+        }
+        if (!L2) {
+            ctx.err = "FFT length of the last axis cannot be split into LDS-resident factors";
+            return NDC_ERR_UNSUPPORTED;
+        }
+    }
+    LinePlan<R> rowpl;
+    if (!big && !make_lineplan<R>(&rowpl, L, tws, ctx)) {
+        ctx.err = "twiddle table allocation failed";
+        return NDC_ERR_OUT_OF_MEMORY;
+    }
+    int64_t offs[kMaxRank + 1];
+    const int32_t* maps = ctx.border_maps(p, offs);
+    if (!maps) return NDC_ERR_OUT_OF_MEMORY;
+
+    PadSrc<E> pad;
+    std::memset(&pad, 0, sizeof(pad));
+    pad.x = static_cast<const E*>(xv);
+    KerSrc<E> ker;
+    std::memset(&ker, 0, sizeof(ker));
+    ker.k = static_cast<const E*>(kv);
+    ker.rev = p.reverse ? 1 : 0;
+    for (int i = 0; i < N; ++i) {
+        pad.map[i] = maps + offs[i];
+        pad.cst[i][0] = host_value<E>(p.side[i][0].value);
+        pad.cst[i][1] = host_value<E>(p.side[i][1].value);
+        pad.xstr[i] = p.x_str[i];
+        ker.kstr[i] = p.k_str[i];
+        ker.K[i] = int32_t(p.k_shape[i]);
+        ker.d[i] = int32_t(p.dil[i]);
+    }
+    double prodF = 1;
+    for (int i = 0; i < N; ++i) prodF *= double(p.fft[i]);
+    const R scale = R(1.0 / prodF);
+    E* y = static_cast<E*>(yv);
+    const double esz = double(sizeof(E));
+
+    // rank 1, line fits three buffers: one fused launch
+    if (N == 1 && !big && size_t(3) * L * es <= lds) {
+        Fused1dArgs<R, E> a;
+        std::memset(&a, 0, sizeof(a));
+        a.pad = pad;
+        a.ker = ker;
+        a.y = y;
+        a.F = Fl;
+        a.H = H;
+        a.mode = mode;
+        a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
+        a.scale = scale;
+        a.pl = rowpl;
+        const size_t smem = size_t(3) * L * es;
+        ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.k_numel + p.y_numel), 15.0 * L * std::log2(double(L)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "fused rank-1 FFT convolution");
+    }
+
+    // spectrum layout [F0, ..., F_{N-2}, Hp]
+    const int64_t Hp = big ? H : (H + 7) / 8 * 8;
+    int64_t sstr[kMaxRank];
+    sstr[last] = 1;
+    if (N >= 2) sstr[last - 1] = Hp;
+    for (int e = last - 2; e >= 0; --e) sstr[e] = sstr[e + 1] * p.fft[e + 1];
+    const int64_t spec_elems = N >= 2 ? sstr[0] * p.fft[0] : Hp;
+    C* Sx = static_cast<C*>(ctx.scratch(0, size_t(spec_elems) * es));
+    C* Sk = static_cast<C*>(ctx.scratch(1, size_t(spec_elems) * es));
+    if (!Sx || !Sk) {
+        ctx.err = "spectrum workspace allocation failed";
+        return NDC_ERR_OUT_OF_MEMORY;
+    }
+    const AxisSel none{0, 1, 1};
+
+    auto rows_fwd = [&](bool data_job) -> ndc_status {
+        RowsFwdArgs<R, E> a;
+        std::memset(&a, 0, sizeof(a));
+        a.pad = pad;
+        a.ker = ker;
+        a.spec[0] = Sx;
+        a.spec[1] = Sk;
+        for (int e = 0; e < last; ++e) {
+            a.sstr[e] = sstr[e];
+            a.cnt[0][e] = p.pds[e];
+            a.cnt[1][e] = p.kdil[e];
+        }
+        a.nlines[0] = data_job ? 1 : 0;
+        a.nlines[1] = 1;
+        for (int e = 0; e < last; ++e) {
+            a.nlines[0] *= a.cnt[0][e];
+            a.nlines[1] *= a.cnt[1][e];
+        }
+        a.nout = last;
+        a.F = Fl;
+        a.H = H;
+        a.mode = mode;
+        a.ls = L;
+        int lpb = int(std::max<int64_t>(1, std::min<int64_t>(16, (2 * kThreads) / std::max(1, L))));
+        while (lpb > 1 && size_t(2) * lpb * L * es > 64 * 1024) lpb /= 2;
+        a.lpb = lpb;
+        const size_t smem = size_t(2) * lpb * L * es;
+        a.pl = rowpl;
+        const int64_t nb0 = (a.nlines[0] + lpb - 1) / lpb;
+        const int64_t nb1 = (a.nlines[1] + lpb - 1) / lpb;
+        a.nblocks0 = int32_t(nb0);
+        const double bytes = esz * double((data_job ? p.x_numel : 0) + p.k_numel) +
+                             double(es) * double(a.nlines[0] + a.nlines[1]) * H;
+        ctx.kbegin("fft_rows_fwd", bytes, 5.0 * double(a.nlines[0] + a.nlines[1]) * L * std::log2(double(L)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_fwd_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((rows_fwd_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "rows forward");
+    };
+
+    // rank 1, line fits two buffers: kernel spectrum first, then the fused data pass
+    if (N == 1 && !big) {
+        ndc_status s = rows_fwd(false);
+        if (s != NDC_OK) return s;
+        Fused1dArgs<R, E> a;
+        std::memset(&a, 0, sizeof(a));
+        a.pad = pad;
+        a.ker = ker;
+        a.ks = Sk;
+        a.y = y;
+        a.F = Fl;
+        a.H = H;
+        a.mode = mode;
+        a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
+        a.scale = scale;
+        a.pl = rowpl;
+        const size_t smem = size_t(2) * L * es;
+        ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.y_numel) + double(es) * H, 10.0 * L * std::log2(double(L)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "fused rank-1 FFT convolution");
+    }
+
+    // four-step passes of the long last axis
+    LinePlan<R> pl1, pl2;
+    const Cx<R>* twL = nullptr;
+    if (big) {
+        if (!make_lineplan<R>(&pl1, L1, 1, ctx) || !make_lineplan<R>(&pl2, L2, 1, ctx) ||
+            !(twL = static_cast<const Cx<R>*>(ctx.twiddles(Fl, sizeof(R) == 8)))) {
+            ctx.err = "twiddle table allocation failed";
+            return NDC_ERR_OUT_OF_MEMORY;
+        }
+    }
+    auto big_col = [&](bool inv) -> ndc_status {
+        BigColArgs<R, E> a;
+        std::memset(&a, 0, sizeof(a));
+        a.pad = pad;
+        a.ker = ker;
+        a.spec[0] = Sx;
+        a.spec[1] = Sk;
+        a.y = y;
+        a.nout = last;
+        int64_t ys = p.out[last];
+        for (int e = last - 1; e >= 0; --e) {
+            a.sstr[e] = sstr[e];
+            a.ystr[e] = ys;
+            ys *= p.out[e];
+            if (inv) a.sel[0][e] = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
+            else {
+                a.sel[0][e] = AxisSel{0, 1, p.pds[e]};
+                a.sel[1][e] = AxisSel{0, 1, p.kdil[e]};
+            }
+        }
+        for (int e = last; e < kMaxRank; ++e) a.sel[0][e] = a.sel[1][e] = none;
+        a.nlines[0] = 1;
+        a.nlines[1] = inv ? 0 : 1;
+        for (int e = 0; e < last; ++e) {
+            a.nlines[0] *= a.sel[0][e].count;
+            if (!inv) a.nlines[1] *= a.sel[1][e].count;
+        }
+        a.L1 = L1;
+        a.L2 = L2;
+        a.Ct = Ctb;
+        a.ntiles = (L2 + Ctb - 1) / Ctb;
+        a.inv = inv ? 1 : 0;
+        a.lastsel = AxisSel{p.kdil[last] - 1, p.stride[last], p.out[last]};
+        a.scale = scale;
+        a.twL = twL;
+        a.pl = pl1;
+        const int64_t nb0 = a.nlines[0] * a.ntiles, nb1 = a.nlines[1] * a.ntiles;
+        a.nblocks0 = int32_t(nb0);
+        const size_t smem = size_t(2) * L1 * Ctb * es;
+        const double lines = double(a.nlines[0] + a.nlines[1]);
+        const double bytes = inv ? double(es) * lines * Fl + esz * double(p.y_numel)
+                                 : esz * double(p.x_numel + p.k_numel) + double(es) * lines * Fl;
+        ctx.kbegin(inv ? "fft_bigcol_inv" : "fft_bigcol_fwd", bytes, 5.0 * lines * Fl * std::log2(double(L1)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigcol_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((bigcol_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "long-axis column pass");
+    };
+    auto big_row = [&](int rmode) -> ndc_status {
+        BigRowArgs<R> a;
+        std::memset(&a, 0, sizeof(a));
+        a.spec[0] = Sx;
+        a.spec[1] = Sk;
+        a.nout = last;
+        a.L1 = L1;
+        a.L2 = L2;
+        a.mode = rmode;
+        a.pl = pl2;
+        for (int e = 0; e < last; ++e) {
+            a.sstr[e] = sstr[e];
+            if (rmode == 2) a.sel[0][e] = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
+            else {
+                a.sel[0][e] = AxisSel{0, 1, rmode == 1 ? 1 : p.pds[e]};
+                a.sel[1][e] = AxisSel{0, 1, p.kdil[e]};
+            }
+        }
+        a.nrows[0] = L1;
+        a.nrows[1] = rmode == 0 ? L1 : 0;
+        for (int e = 0; e < last; ++e) {
+            a.nrows[0] *= a.sel[0][e].count;
+            if (rmode == 0) a.nrows[1] *= a.sel[1][e].count;
+        }
+        a.nblocks0 = int32_t(a.nrows[0]);
+        const size_t smem = size_t(rmode == 1 ? 3 : 2) * L2 * es;
+        const double rows = double(a.nrows[0] + a.nrows[1]);
+        const double bytes = double(es) * L2 * (rmode == 1 ? 3.0 * a.nrows[0] : 2.0 * rows);
+        static const char* names[3] = {"fft_bigrow_fwd", "fft_bigrow_fused", "fft_bigrow_inv"};
+        ctx.kbegin(names[rmode], bytes, 5.0 * rows * (rmode == 1 ? 3 : 1) * L2 * std::log2(double(L2)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigrow_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((bigrow_kernel<R>), dim3(unsigned(a.nrows[0] + a.nrows[1])), dim3(kThreads), smem,
+                           ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "long-axis row pass");
+    };
+
+    if (N == 1) {   // big rank-1 line
+        ndc_status s = big_col(false);
+        if (s == NDC_OK) s = big_row(1);
+        if (s == NDC_OK) s = big_col(true);
+        return s;
+    }
+
+    // ---- pass 1: last axis forward
+    if (big) {
+        ndc_status s = big_col(false);
+        if (s == NDC_OK) s = big_row(0);
+        if (s != NDC_OK) return s;
+    } else {
+        ndc_status s = rows_fwd(true);
+        if (s != NDC_OK) return s;
+    }
+
+    auto col_pass = [&](int d, int cmode) -> ndc_status {
+        ColsArgs<R> a;
+        std::memset(&a, 0, sizeof(a));
+        a.spec[0] = Sx;
+        a.spec[1] = Sk;
+        for (int e = 0; e < N; ++e) a.sstr[e] = sstr[e];
+        a.d = d;
+        a.H = H;
+        a.mode = cmode;
+        if (!make_lineplan<R>(&a.pl, int(p.fft[d]), 1, ctx)) {
+            ctx.err = "twiddle table allocation failed";
+            return NDC_ERR_OUT_OF_MEMORY;
+        }
+        const int Ld = int(p.fft[d]);
+        int no = 0;
+        for (int e = 0; e < last; ++e) {
+            if (e == d) continue;
+            a.outer_axis[no] = e;
+            for (int j = 0; j < 2; ++j) {
+                AxisSel s{0, 1, p.fft[e]};
+                if (cmode == 0 && e < d) s = AxisSel{0, 1, j == 0 ? p.pds[e] : p.kdil[e]};
+                if (cmode == 2 && e < d) s = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
+                a.outer[j][no] = s;
+            }
+            ++no;
+        }
+        a.nouter = no;
+        a.nlines_outer[0] = a.nlines_outer[1] = 1;
+        for (int i = 0; i < no; ++i) {
+            a.nlines_outer[0] *= a.outer[0][i].count;
+            a.nlines_outer[1] *= a.outer[1][i].count;
+        }
+        a.valid[0] = int(cmode == 2 ? Ld : p.pds[d]);
+        a.valid[1] = int(p.kdil[d]);
+        if (cmode == 0) a.outsel = AxisSel{0, 1, Ld};
+        else a.outsel = AxisSel{p.kdil[d] - 1, p.stride[d], p.out[d]};
+        const int nbuf = cmode == 1 ? 3 : 2;
+        // tile width: enough lines for coalescing, enough blocks to fill the chip
+        int Ct = 16;
+        while (Ct > 1 && size_t(nbuf) * Ld * Ct * es > 64 * 1024) Ct /= 2;
+        while (Ct > 4 && a.nlines_outer[0] * ((H + Ct - 1) / Ct) < 2 * ctx.cu_count()) Ct /= 2;
+        const size_t smem = size_t(nbuf) * Ld * Ct * es;
+        if (smem > lds) {
+            ctx.err = "FFT length of a column axis exceeds the LDS-resident line size";
+            return NDC_ERR_UNSUPPORTED;
+        }
+        a.C = Ct;
+        a.ntiles = (H + Ct - 1) / Ct;
+        const int64_t nb0 = a.nlines_outer[0] * a.ntiles;
+        const int64_t nb1 = cmode == 0 ? a.nlines_outer[1] * a.ntiles : 0;
+        a.nblocks0 = int32_t(nb0);
+        // compulsory traffic: the rows read (valid / all) and the rows written
+        double lines0 = double(a.nlines_outer[0]) * H, lines1 = double(a.nlines_outer[1]) * H;
+        double bytes = 0, nfft = 0;
+        if (cmode == 0) { bytes = (lines0 * (a.valid[0] + Ld) + lines1 * (a.valid[1] + Ld)); nfft = lines0 + lines1; }
+        else if (cmode == 1) { bytes = lines0 * (a.valid[0] + a.valid[1] + double(a.outsel.count)); nfft = 3 * lines0; }
+        else { bytes = lines0 * (Ld + double(a.outsel.count)); nfft = lines0; }
+        bytes *= double(es);
+        static const char* names[3] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv"};
+        ctx.kbegin(names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&cols_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((cols_kernel<R>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        return check_launch(ctx, "column pass");
+    };
+
+    // ---- pass 2: axes N-2 .. 1 forward
+    for (int d = last - 1; d >= 1; --d) {
+        ndc_status s = col_pass(d, 0);
+        if (s != NDC_OK) return s;
+    }
+    // ---- pass 3: axis 0 fused
+    {
+        ndc_status s = col_pass(0, 1);
+        if (s != NDC_OK) return s;
+    }
+    // ---- pass 4: axes 1 .. N-2 inverse
+    for (int d = 1; d <= last - 1; ++d) {
+        ndc_status s = col_pass(d, 2);
+        if (s != NDC_OK) return s;
+    }
+    // ---- pass 5: last axis inverse with crop
+    if (big) {
+        ndc_status s = big_row(2);
+        if (s == NDC_OK) s = big_col(true);
+        return s;
+    }
+    {
+        RowsInvArgs<R, E> a;
+        std::memset(&a, 0, sizeof(a));
+        a.spec = Sx;
+        a.y = y;
+        a.nout = last;
+        int64_t ys = p.out[last];
+        for (int e = last - 1; e >= 0; --e) {
+            a.ystr[e] = ys;
+            ys *= p.out[e];
+        }
+        a.nlines = 1;
+        for (int e = 0; e < last; ++e) {
+            a.sstr[e] = sstr[e];
+            a.sel[e] = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
+            a.nlines *= p.out[e];
+        }
+        a.F = Fl;
+        a.H = H;
+        a.mode = mode;
+        a.ls = L;
+        int lpb = int(std::max<int64_t>(1, std::min<int64_t>(16, (2 * kThreads) / std::max(1, L))));
+        while (lpb > 1 && size_t(2) * lpb * L * es > 64 * 1024) lpb /= 2;
+        a.lpb = lpb;
+        a.lastsel = AxisSel{p.kdil[last] - 1, p.stride[last], p.out[last]};
+        a.scale = scale;
+        a.pl = rowpl;
+        const size_t smem = size_t(2) * lpb * L * es;
+        const int64_t nb = (a.nlines + lpb - 1) / lpb;
+        const double bytes = double(es) * double(a.nlines) * H + esz * double(p.y_numel);
+        ctx.kbegin("fft_rows_inv", bytes, 5.0 * double(a.nlines) * L * std::log2(double(L)));
+        if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_inv_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL((rows_inv_kernel<R, E>), dim3(unsigned(nb)), dim3(kThreads), smem, ctx.stream(), a);
+        ctx.kend();
+        ndc_status s = check_launch(ctx, "rows inverse");
+        if (s != NDC_OK) return s;
+    }
+    return NDC_OK;
+}
+
+}  // namespace
+
+ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx) {
+    switch (p.dtype) {
+        case NDC_F32: return run_fft<float, float>(p, x, k, y, ctx);
+        case NDC_F64: return run_fft<double, double>(p, x, k, y, ctx);
+        case NDC_C64: return run_fft<float, float2>(p, x, k, y, ctx);
+        case NDC_C128: return run_fft<double, double2>(p, x, k, y, ctx);
+        default: return NDC_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace ndc

@@ -1,0 +1,68 @@
+// Interface between the C-ABI layer (capi.cpp) and the kernel launchers in
+// the .hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+
+#include "plan.hpp"
+
+namespace ndc {
+
+// LDS bytes kept free for the kernels' static __shared__ arrays when sizing
+// dynamic LDS (the two share one per-workgroup limit).
+constexpr size_t kStaticLdsReserve = 1024;
+
+// Services a launcher needs from the handle that owns the call.
+class LaunchContext {
+public:
+    virtual ~LaunchContext() = default;
+    virtual hipStream_t stream() = 0;
+    // Device scratch of at least `bytes`, 256-byte aligned, valid for the
+    // duration of the current call (slot 0..3 are independent buffers).
+    virtual void* scratch(int slot, size_t bytes) = 0;
+    // Device table of W_n^i = exp(-2*pi*i*I/n), i < n, generated in FP64 on
+    // the host and stored as float2 (dbl = false) or double2 (dbl = true).
+    virtual const void* twiddles(int64_t n, bool dbl) = 0;
+    // Device copy of the plan's border maps, concatenated per axis.
+    // offsets[i] is the element offset of axis i's map; the entry at
+    // offsets[kMaxRank] is a single 0 used for virtual unit axes.
+    virtual const int32_t* border_maps(const Plan& p, int64_t offsets[kMaxRank + 1]) = 0;
+    virtual int cu_count() = 0;
+    // Largest dynamic LDS allocation one workgroup may request.
+    virtual size_t max_lds() = 0;
+    // Bracket one kernel launch for the optional per-kernel profile: `name`
+    // identifies the kernel, `bytes` is its algorithmic HBM traffic (compulsory
+    // reads + writes of that pass) and `flops` its algorithmic arithmetic.
+    virtual void kbegin(const char* name, double bytes, double flops) { (void)name; (void)bytes; (void)flops; }
+    virtual void kend() {}
+    std::string err;
+};
+
+// True when `kernel`'s static LDS plus `dynamic_bytes` fits one workgroup;
+// a launch that would not fit is refused on the host instead of reaching the
+// device.  The static size is looked up once per kernel.
+inline bool lds_fits(LaunchContext& ctx, const void* kernel, size_t dynamic_bytes) {
+    thread_local std::unordered_map<const void*, size_t> cache;
+    auto it = cache.find(kernel);
+    if (it == cache.end()) {
+        hipFuncAttributes attr;
+        if (hipFuncGetAttributes(&attr, kernel) != hipSuccess) return false;
+        it = cache.emplace(kernel, size_t(attr.sharedSizeBytes)).first;
+    }
+    if (it->second + dynamic_bytes > ctx.max_lds() + kStaticLdsReserve) {
+        ctx.err = "kernel LDS request exceeds the workgroup limit";
+        return false;
+    }
+    return true;
+}
+
+// Direct convolution (conv/mod.rs:128-200) in the element type.
+ndc_status launch_direct(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
+
+// FFT convolution (conv_fft/mod.rs:185-292).
+ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
+
+}  // namespace ndc

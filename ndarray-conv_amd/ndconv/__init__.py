@@ -1,0 +1,483 @@
+"""Host-side mirror of ndarray-conv's convolution API over the HIP C ABI.
+
+The names, argument meaning and error behaviour follow the reference crate
+(ndarray-conv 0.6.1):
+
+    conv(data, kernel, conv_mode, padding_mode)          ConvExt::conv            conv/mod.rs:110-115
+    conv_fft(data, kernel, conv_mode, padding_mode)      ConvFFTExt::conv_fft     conv_fft/mod.rs:113-118
+    conv_fft_with_processor(..., processor)                                       conv_fft/mod.rs:151-157
+    conv_fft_par(...)                                                             conv_fft/mod.rs:166-171
+    ConvMode.Full/Same/Valid/Custom/Explicit                                      lib.rs:80-105
+    PaddingMode.Zeros/Const/Reflect/Replicate/Circular/Custom/Explicit            lib.rs:106-127
+    BorderType.Zeros/Const/Reflect/Replicate/Circular                             lib.rs:129-143
+    with_dilation(kernel, d), reverse(k), no_reverse(k)                           dilation/mod.rs:103-189
+    DataShape / KernelShape / MismatchShape errors                                lib.rs:145-159
+
+`data` and `kernel` are numpy arrays (host memory, copied to and from the
+GPU inside the call) or torch tensors already on the GPU (HBM-resident; the
+result is a new GPU tensor and the call is ordered on torch's current
+stream).  Reference panics (stride 0, reflect pads leaving the axis) raise
+InvalidArgument.  Every computation runs in libndconv.so; there is no CPU
+fallback.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib as C
+
+__all__ = ["ConvMode", "PaddingMode", "BorderType", "KernelWithDilation", "with_dilation", "reverse",
+           "no_reverse", "conv", "conv_fft", "conv_fft_with_processor", "conv_fft_par", "get_fft_processor",
+           "FftProcessor", "plan", "border_map", "Error", "DataShape", "KernelShape", "MismatchShape",
+           "InvalidArgument", "Unsupported", "DeviceError"]
+
+
+# ---------------------------------------------------------------------------
+# Errors (lib.rs:145-159)
+# ---------------------------------------------------------------------------
+
+class Error(Exception):
+    """Base of the convolution errors."""
+
+
+class DataShape(Error):
+    """Data shape shouldn't have ZERO (Error::DataShape)."""
+
+
+class KernelShape(Error):
+    """Kernel shape shouldn't have ZERO (Error::KernelShape)."""
+
+
+class MismatchShape(Error):
+    """ConvMode does not match KernelWithDilation size (Error::MismatchShape)."""
+
+
+class InvalidArgument(Error):
+    """An input on which the reference panics (stride 0, reflect pad out of range)."""
+
+
+class Unsupported(Error):
+    pass
+
+
+class DeviceError(Error):
+    pass
+
+
+_STATUS_EXC = {C.ERR_DATA_SHAPE: DataShape, C.ERR_KERNEL_SHAPE: KernelShape,
+               C.ERR_MISMATCH_SHAPE: MismatchShape, C.ERR_INVALID_ARGUMENT: InvalidArgument,
+               C.ERR_UNSUPPORTED: Unsupported, C.ERR_NO_DEVICE: DeviceError, C.ERR_DEVICE: DeviceError,
+               C.ERR_OUT_OF_MEMORY: DeviceError}
+
+
+def _check(status, handle=None):
+    if status == C.OK:
+        return
+    lib = C.lib()
+    msg = lib.ndc_status_string(status).decode()
+    if handle is not None:
+        detail = lib.ndc_last_error(handle).decode()
+        if detail:
+            msg = f"{msg}: {detail}"
+    raise _STATUS_EXC.get(status, Error)(msg)
+
+
+# ---------------------------------------------------------------------------
+# Modes (lib.rs:80-143)
+# ---------------------------------------------------------------------------
+
+class _ConvModeValue:
+    __slots__ = ("tag", "padding", "strides", "name")
+
+    def __init__(self, tag, name, padding=None, strides=None):
+        self.tag, self.name, self.padding, self.strides = tag, name, padding, strides
+
+    def __repr__(self):
+        if self.padding is None:
+            return f"ConvMode.{self.name}"
+        return f"ConvMode.{self.name}(padding={self.padding}, strides={self.strides})"
+
+
+class ConvMode:
+    Full = _ConvModeValue(C.CONV_FULL, "Full")
+    Same = _ConvModeValue(C.CONV_SAME, "Same")
+    Valid = _ConvModeValue(C.CONV_VALID, "Valid")
+
+    @staticmethod
+    def Custom(padding, strides):
+        return _ConvModeValue(C.CONV_CUSTOM, "Custom", [int(p) for p in padding], [int(s) for s in strides])
+
+    @staticmethod
+    def Explicit(padding, strides):
+        return _ConvModeValue(C.CONV_EXPLICIT, "Explicit", [[int(a), int(b)] for a, b in padding],
+                              [int(s) for s in strides])
+
+
+class _BorderValue:
+    __slots__ = ("tag", "value", "name")
+
+    def __init__(self, tag, name, value=None):
+        self.tag, self.name, self.value = tag, name, value
+
+    def __repr__(self):
+        return f"BorderType.{self.name}" + ("" if self.value is None else f"({self.value})")
+
+
+class BorderType:
+    Zeros = _BorderValue(C.BORDER_ZEROS, "Zeros")
+    Reflect = _BorderValue(C.BORDER_REFLECT, "Reflect")
+    Replicate = _BorderValue(C.BORDER_REPLICATE, "Replicate")
+    Circular = _BorderValue(C.BORDER_CIRCULAR, "Circular")
+
+    @staticmethod
+    def Const(c):
+        return _BorderValue(C.BORDER_CONST, "Const", c)
+
+
+class _PaddingValue:
+    __slots__ = ("tag", "value", "borders", "name")
+
+    def __init__(self, tag, name, value=None, borders=None):
+        self.tag, self.name, self.value, self.borders = tag, name, value, borders
+
+    def __repr__(self):
+        if self.borders is not None:
+            return f"PaddingMode.{self.name}({self.borders})"
+        return f"PaddingMode.{self.name}" + ("" if self.value is None else f"({self.value})")
+
+
+class PaddingMode:
+    Zeros = _PaddingValue(C.PAD_ZEROS, "Zeros")
+    Reflect = _PaddingValue(C.PAD_REFLECT, "Reflect")
+    Replicate = _PaddingValue(C.PAD_REPLICATE, "Replicate")
+    Circular = _PaddingValue(C.PAD_CIRCULAR, "Circular")
+
+    @staticmethod
+    def Const(c):
+        return _PaddingValue(C.PAD_CONST, "Const", value=c)
+
+    @staticmethod
+    def Custom(borders):
+        return _PaddingValue(C.PAD_CUSTOM, "Custom", borders=list(borders))
+
+    @staticmethod
+    def Explicit(borders):
+        return _PaddingValue(C.PAD_EXPLICIT, "Explicit", borders=[list(b) for b in borders])
+
+
+# ---------------------------------------------------------------------------
+# Kernel wrappers (dilation/mod.rs:8-12, 103-212)
+# ---------------------------------------------------------------------------
+
+class KernelWithDilation:
+    """A kernel with per-axis dilation and a reverse flag (True = convolution,
+    False = cross-correlation)."""
+    __slots__ = ("kernel", "dilation", "reverse_flag")
+
+    def __init__(self, kernel, dilation=1, reverse_flag=True):
+        self.kernel = kernel
+        nd = len(kernel.shape)
+        if np.isscalar(dilation) or isinstance(dilation, (int, np.integer)):
+            dilation = [int(dilation)] * nd
+        self.dilation = [int(d) for d in dilation]
+        self.reverse_flag = bool(reverse_flag)
+
+    def with_dilation(self, dilation):
+        return KernelWithDilation(self.kernel, dilation, self.reverse_flag)
+
+    def reverse(self):
+        return KernelWithDilation(self.kernel, self.dilation, True)
+
+    def no_reverse(self):
+        return KernelWithDilation(self.kernel, self.dilation, False)
+
+
+def with_dilation(kernel, dilation):
+    """WithDilation::with_dilation: a usize for every axis or one per axis; reverse stays on."""
+    return KernelWithDilation(kernel, dilation, True)
+
+
+def reverse(kernel):
+    return _into_kwd(kernel).reverse()
+
+
+def no_reverse(kernel):
+    return _into_kwd(kernel).no_reverse()
+
+
+def _into_kwd(k):
+    if isinstance(k, KernelWithDilation):
+        return k
+    return KernelWithDilation(k, 1, True)
+
+
+# ---------------------------------------------------------------------------
+# Descriptor packing
+# ---------------------------------------------------------------------------
+
+_NP_DTYPE = {np.dtype(np.float32): C.F32, np.dtype(np.float64): C.F64, np.dtype(np.complex64): C.C64,
+             np.dtype(np.complex128): C.C128, np.dtype(np.int8): C.I8, np.dtype(np.int16): C.I16,
+             np.dtype(np.int32): C.I32, np.dtype(np.int64): C.I64, np.dtype(np.uint8): C.U8,
+             np.dtype(np.uint16): C.U16, np.dtype(np.uint32): C.U32, np.dtype(np.uint64): C.U64}
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _np_dtype_of(a):
+    if _is_torch(a):
+        import torch
+        m = {torch.float32: np.float32, torch.float64: np.float64, torch.complex64: np.complex64,
+             torch.complex128: np.complex128, torch.int8: np.int8, torch.int16: np.int16,
+             torch.int32: np.int32, torch.int64: np.int64, torch.uint8: np.uint8}
+        if a.dtype not in m:
+            raise Unsupported(f"dtype {a.dtype}")
+        return np.dtype(m[a.dtype])
+    return np.asarray(a).dtype
+
+
+def _value(v, dtype):
+    val = C.Value()
+    if v is None:
+        return val
+    if dtype in (C.F32, C.F64):
+        val.re = float(np.real(v))
+    elif dtype in (C.C64, C.C128):
+        val.re, val.im = float(np.real(v)), float(np.imag(v))
+    else:
+        iv = int(v)
+        if iv >= 2 ** 63:
+            iv -= 2 ** 64
+        val.i = iv
+    return val
+
+
+def _fill_desc(d, x, kwd, conv_mode, padding_mode, dtype, host):
+    k = kwd.kernel
+    n = len(x.shape)
+    if len(k.shape) != n:
+        raise InvalidArgument("data and kernel ranks differ")
+    if n < 1 or n > C.MAX_RANK:
+        raise Unsupported(f"rank {n}")
+    d.dtype = dtype
+    d.rank = n
+    flags = C.FLAG_HOST_MEMORY if host else 0
+    for arr, shp, strd, flag in ((x, d.x_shape, d.x_strides, C.FLAG_X_STRIDED),
+                                 (k, d.k_shape, d.k_strides, C.FLAG_K_STRIDED)):
+        for i, s in enumerate(arr.shape):
+            shp[i] = int(s)
+        if _is_torch(arr):
+            st = arr.stride()
+        else:
+            st = [s // arr.itemsize for s in arr.strides]
+            if any(s % arr.itemsize for s in arr.strides):
+                raise Unsupported("strides that are not a multiple of the element size")
+        for i, s in enumerate(st):
+            strd[i] = int(s)
+        flags |= flag
+    if len(kwd.dilation) != n:
+        raise InvalidArgument("dilation rank differs from the kernel rank")
+    for i, dl in enumerate(kwd.dilation):
+        d.dilation[i] = dl
+    d.reverse = 1 if kwd.reverse_flag else 0
+    cm = d.conv_mode
+    cm.tag = conv_mode.tag
+    if conv_mode.tag == C.CONV_CUSTOM:
+        if len(conv_mode.padding) != n or len(conv_mode.strides) != n:
+            raise InvalidArgument("ConvMode::Custom rank differs")
+        for i in range(n):
+            cm.padding[i][0] = cm.padding[i][1] = conv_mode.padding[i]
+            cm.strides[i] = conv_mode.strides[i]
+    elif conv_mode.tag == C.CONV_EXPLICIT:
+        if len(conv_mode.padding) != n or len(conv_mode.strides) != n:
+            raise InvalidArgument("ConvMode::Explicit rank differs")
+        for i in range(n):
+            cm.padding[i][0], cm.padding[i][1] = conv_mode.padding[i]
+            cm.strides[i] = conv_mode.strides[i]
+    pm = d.padding_mode
+    pm.tag = padding_mode.tag
+    pm.value = _value(padding_mode.value, dtype)
+    if padding_mode.tag == C.PAD_CUSTOM:
+        if len(padding_mode.borders) != n:
+            raise InvalidArgument("PaddingMode::Custom rank differs")
+        for i, b in enumerate(padding_mode.borders):
+            pm.borders[i][0].tag = b.tag
+            pm.borders[i][0].value = _value(b.value, dtype)
+    elif padding_mode.tag == C.PAD_EXPLICIT:
+        if len(padding_mode.borders) != n:
+            raise InvalidArgument("PaddingMode::Explicit rank differs")
+        for i, (b0, b1) in enumerate(padding_mode.borders):
+            pm.borders[i][0].tag = b0.tag
+            pm.borders[i][0].value = _value(b0.value, dtype)
+            pm.borders[i][1].tag = b1.tag
+            pm.borders[i][1].value = _value(b1.value, dtype)
+    d.flags = flags
+
+
+def _desc_for_shapes(xshape, kshape, conv_mode, padding_mode, dilation=1, reverse_flag=True, dtype=np.float32):
+    """Descriptor from shapes only (plan queries)."""
+    class _S:
+        def __init__(self, shape, itemsize):
+            self.shape = tuple(shape)
+            self.itemsize = itemsize
+            st, acc = [], itemsize
+            for s in reversed(self.shape):
+                st.append(acc)
+                acc *= max(int(s), 1)
+            self.strides = tuple(reversed(st))
+    dt = np.dtype(dtype)
+    x = _S(xshape, dt.itemsize)
+    k = _S(kshape, dt.itemsize)
+    d = C.ConvDesc()
+    _fill_desc(d, x, KernelWithDilation(k, dilation, reverse_flag), conv_mode, padding_mode, _NP_DTYPE[dt], True)
+    return d
+
+
+# ---------------------------------------------------------------------------
+# Processor / handle (processor/mod.rs:71-143)
+# ---------------------------------------------------------------------------
+
+class FftProcessor:
+    """Owns a library handle: HIP stream, device workspace, twiddle tables and
+    plan caches.  Reusing one across calls is what conv_fft_with_processor is
+    for.  Not thread-safe (use one per thread, as &mut Processor)."""
+
+    def __init__(self):
+        lib = C.lib()
+        h = ctypes.c_void_p()
+        _check(lib.ndc_create(ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        _check(C.lib().ndc_synchronize(self._h), self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            C.lib().ndc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_fft_processor():
+    """get_fft_processor (processor/mod.rs:71-73)."""
+    return FftProcessor()
+
+
+_tls = threading.local()
+
+
+def _default_processor():
+    p = getattr(_tls, "proc", None)
+    if p is None:
+        p = FftProcessor()
+        _tls.proc = p
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Operations
+# ---------------------------------------------------------------------------
+
+def plan(data_shape, kernel_shape, conv_mode, padding_mode=PaddingMode.Zeros, dilation=1, reverse_flag=True,
+         fft=False, dtype=np.float32):
+    """Shape arithmetic of one convolution (padding, strides, kdil, padded, fft_size, out_shape)."""
+    d = _desc_for_shapes(data_shape, kernel_shape, conv_mode, padding_mode, dilation, reverse_flag, dtype)
+    info = C.PlanInfo()
+    _check(C.lib().ndc_plan(ctypes.byref(d), C.OP_CONV_FFT if fft else C.OP_CONV, ctypes.byref(info)))
+    n = len(data_shape)
+    return dict(padding=[[info.padding[i][0], info.padding[i][1]] for i in range(n)],
+                strides=[info.strides[i] for i in range(n)],
+                kernel_dilated=[info.kernel_dilated[i] for i in range(n)],
+                padded=[info.padded[i] for i in range(n)],
+                fft_size=[info.fft_size[i] for i in range(n)],
+                out_shape=[info.out_shape[i] for i in range(n)])
+
+
+def border_map(data_shape, kernel_shape, conv_mode, padding_mode, axis, dilation=1, fft=False):
+    """The resolved border map of one axis (see ndc_border_map)."""
+    info = plan(data_shape, kernel_shape, conv_mode, padding_mode, dilation, True, fft)
+    n = info["fft_size"][axis] if fft else info["padded"][axis]
+    d = _desc_for_shapes(data_shape, kernel_shape, conv_mode, padding_mode, dilation, True)
+    out = np.zeros(n, dtype=np.int32)
+    _check(C.lib().ndc_border_map(ctypes.byref(d), C.OP_CONV_FFT if fft else C.OP_CONV, axis,
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n))
+    return out
+
+
+_ALGOS = {"auto": C.ALGO_AUTO, "direct": C.ALGO_DIRECT, "bf16": C.ALGO_MFMA_BF16, "bf16x3": C.ALGO_MFMA_BF16X3}
+
+
+def _run(op, data, kernel, conv_mode, padding_mode, processor=None, algo="auto"):
+    lib = C.lib()
+    kwd = _into_kwd(kernel)
+    proc = processor or _default_processor()
+    torch_in = _is_torch(data) or _is_torch(kwd.kernel)
+    if torch_in:
+        import torch
+        if not (_is_torch(data) and _is_torch(kwd.kernel)) or not data.is_cuda or not kwd.kernel.is_cuda:
+            raise InvalidArgument("torch inputs must both be GPU tensors")
+        if data.dtype != kwd.kernel.dtype:
+            raise InvalidArgument("data and kernel dtypes differ")
+        x, k = data, kwd.kernel
+    else:
+        x = np.asarray(data)
+        k = np.asarray(kwd.kernel)
+        if x.dtype != k.dtype:
+            raise InvalidArgument("data and kernel dtypes differ")
+    dt = _np_dtype_of(x)
+    if dt not in _NP_DTYPE:
+        raise Unsupported(f"dtype {dt}")
+    dtype = _NP_DTYPE[dt]
+    d = C.ConvDesc()
+    _fill_desc(d, x, KernelWithDilation(k, kwd.dilation, kwd.reverse_flag), conv_mode, padding_mode, dtype,
+               host=not torch_in)
+    d.algo = _ALGOS[algo]
+    info = C.PlanInfo()
+    _check(lib.ndc_plan(ctypes.byref(d), op, ctypes.byref(info)))
+    out_shape = tuple(int(info.out_shape[i]) for i in range(len(x.shape)))
+    h = proc.handle
+    fn = lib.ndc_conv if op == C.OP_CONV else lib.ndc_conv_fft
+    if torch_in:
+        y = torch.empty(out_shape, dtype=x.dtype, device=x.device)
+        _check(lib.ndc_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)), h)
+        _check(fn(h, ctypes.byref(d), ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(k.data_ptr()),
+                  ctypes.c_void_p(y.data_ptr())), h)
+        return y
+    y = np.empty(out_shape, dtype=dt)
+    _check(lib.ndc_reset_stream(h), h)
+    _check(fn(h, ctypes.byref(d), ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(k.ctypes.data),
+              ctypes.c_void_p(y.ctypes.data)), h)
+    return y
+
+
+def conv(data, kernel, conv_mode, padding_mode, algo="auto", processor=None):
+    """ConvExt::conv (conv/mod.rs:110-115).  algo: "auto"/"direct" (exact, in
+    the element type), "bf16" or "bf16x3" (f32 rank 2 on the matrix cores)."""
+    return _run(C.OP_CONV, data, kernel, conv_mode, padding_mode, processor, algo)
+
+
+def conv_fft(data, kernel, conv_mode, padding_mode):
+    """ConvFFTExt::conv_fft (conv_fft/mod.rs:113-118)."""
+    return _run(C.OP_CONV_FFT, data, kernel, conv_mode, padding_mode)
+
+
+def conv_fft_with_processor(data, kernel, conv_mode, padding_mode, fft_processor):
+    """ConvFFTExt::conv_fft_with_processor (conv_fft/mod.rs:151-157)."""
+    return _run(C.OP_CONV_FFT, data, kernel, conv_mode, padding_mode, fft_processor)
+
+
+def conv_fft_par(data, kernel, conv_mode, padding_mode):
+    """ConvFFTExt::conv_fft_par (conv_fft/mod.rs:166-171): the same transform;
+    the GPU grid replaces rayon."""
+    return _run(C.OP_CONV_FFT, data, kernel, conv_mode, padding_mode)
