@@ -52,6 +52,7 @@ public:
         for (auto& s : io) if (s.ptr) (void)hipFree(s.ptr);
         for (auto& t : tw) (void)hipFree(t.second.ptr);
         for (auto& m : maps) (void)hipFree(m.second.first.ptr);
+        for (auto& t : tables) (void)hipFree(t.second.ptr);
         for (auto& r : recs) { pool.push_back(r.a); pool.push_back(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
         if (own) (void)hipStreamDestroy(own);
@@ -137,6 +138,25 @@ public:
         return b.ptr;
     }
 
+    std::unordered_map<std::string, DevBuf> tables;
+    std::unordered_map<std::string, std::string> memos;
+
+    std::string& memo(const std::string& key) override { return memos[key]; }
+
+    const void* constant_table(const std::string& key, const void* data, size_t bytes) override {
+        auto it = tables.find(key);
+        if (it != tables.end()) return it->second.ptr;
+        DevBuf b;
+        if (hipMalloc(&b.ptr, bytes ? bytes : 16) != hipSuccess) return nullptr;
+        b.bytes = bytes;
+        if (bytes && hipMemcpy(b.ptr, data, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(b.ptr);
+            return nullptr;
+        }
+        tables[key] = b;
+        return b.ptr;
+    }
+
     const int32_t* border_maps(const Plan& p, int64_t offsets[kMaxRank + 1]) override {
         const std::string key = plan_key(p);
         auto it = maps.find(key);
@@ -186,8 +206,9 @@ ndc_status run(ndc_handle hh, const ndc_conv_desc* d, int op, const void* x, con
     const Plan& p = pit->second;
     ndc_status s = NDC_OK;
     if (!x || !k || !y) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "null data pointer");
-    if (op == NDC_OP_CONV && p.algo != NDC_ALGO_AUTO && p.algo != NDC_ALGO_DIRECT)
-        return set_err(hh, NDC_ERR_UNSUPPORTED, "algorithm not available in this build");
+    const bool mfma = op == NDC_OP_CONV && (p.algo == NDC_ALGO_MFMA_BF16 || p.algo == NDC_ALGO_MFMA_BF16X3);
+    if (op == NDC_OP_CONV && !mfma && p.algo != NDC_ALGO_AUTO && p.algo != NDC_ALGO_DIRECT)
+        return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "unknown algorithm");
     if (hipSetDevice(h.device) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "hipSetDevice failed");
 
     const size_t es = dtype_size(p.dtype);
@@ -216,7 +237,9 @@ ndc_status run(ndc_handle hh, const ndc_conv_desc* d, int op, const void* x, con
     }
     h.err.clear();
     if (p.y_numel > 0) {
-        s = op == NDC_OP_CONV ? launch_direct(p, dx, dk, dy, h) : launch_fft(p, dx, dk, dy, h);
+        if (op == NDC_OP_CONV_FFT) s = launch_fft(p, dx, dk, dy, h);
+        else if (mfma) s = launch_direct_mfma(p, dx, dk, dy, h);
+        else s = launch_direct(p, dx, dk, dy, h);
         if (s != NDC_OK) return set_err(hh, s, h.err);
     }
     if (host) {

@@ -4,8 +4,9 @@
 // pass reads x through the per-axis border maps (conv_fft/padding.rs:30-62,
 // padding/half_dim.rs) and k through the dilation/reversal scatter rule
 // (conv_fft/padding.rs:78-111) while loading rows into LDS.  Transforms are
-// mixed-radix Stockham FFTs over lines held in LDS, with FP64-generated
-// twiddles.  Axis order and layout:
+// mixed-radix Stockham FFTs over lines held in LDS.  Each line plan carries a
+// compact twiddle table generated in FP64 on the host; a workgroup stages it
+// into LDS once, so no stage touches global memory.  Axis order and layout:
 //
 //   pass 1  rows (last axis): r2c by the half-length trick (even F) or a full
 //           complex FFT (odd F / complex input), data and kernel in one grid;
@@ -16,12 +17,14 @@
 //   pass 5  rows inverse (c2r), scale 1/prod F (real.rs:278-279,
 //           complex.rs:141-142), crop and stride on store.
 // A rank-1 problem runs as one fused kernel.  Rows beyond the padded extent
-// are known to be zero and are neither read nor written.
+// are known to be zero and are neither read nor written.  Lines longer than
+// LDS take a four-step path (see below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "device_common.hpp"
@@ -30,19 +33,32 @@
 namespace ndc {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kMaxThreads = 1024;
 constexpr int kMaxStages = 24;
+constexpr int kU = 4;   // elements per thread per batched global-load round
 
 __device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 
+// a / d for a < 2^24 and the float reciprocal r of d: the float quotient is
+// within one of the true one, and one correction step fixes it.
+__device__ __forceinline__ int fdiv(int a, int d, float r) {
+    int q = int(float(a) * r);
+    const int rem = a - q * d;
+    if (rem < 0) --q;
+    else if (rem >= d) ++q;
+    return q;
+}
+
 template <typename R>
 struct LinePlan {
-    int L;                 // complex length transformed
-    int nst;               // number of Stockham stages
-    int tws;               // twiddle table holds T = L * tws roots: W_T^i
-    int T;
-    const Cx<R>* tw;
+    int L;                  // complex length transformed
+    int nst;                // Stockham stages
+    int ntw;                // entries of the compact twiddle table
+    int post;               // offset of W_F^k, k = 0..L, for the real-even post/pre-processing; -1 if none
+    const Cx<R>* tw;        // compact table in global memory
     int rad[kMaxStages];
+    int base[kMaxStages];   // stage twiddles W_{Ns*P}^(r*k) at base + k*(P-1) + r-1
+    int wp[kMaxStages];     // radix-P roots W_P^m at wp + m (stages without a fixed butterfly)
 };
 
 // ---------------------------------------------------------------------------
@@ -97,7 +113,7 @@ __device__ __forceinline__ void bfly8(C* v, bool inv) {
     }
 }
 
-// cos/sin(2*pi*m/P), m = 1 .. (P-1)/2
+// cos/sin(2*pi*m/P), m = 0 .. (P-1)/2
 template <int P> struct OddConst;
 template <> struct OddConst<3> {
     static constexpr double c[2] = {1.0, -0.5};
@@ -160,33 +176,37 @@ __device__ __forceinline__ void bfly(C* v, bool inv) {
 // ---------------------------------------------------------------------------
 // Stockham stages over lines in LDS: element (line b, index t) at b*ls + t*es.
 // LINE_FAST selects the thread->item order: line index fastest for column
-// tiles (es = #lines, consecutive lines adjacent), element index fastest for
-// rows (es = 1).
+// tiles (es = #lines, consecutive lines adjacent in LDS), element index
+// fastest for rows (es = 1).  `tw` points at the plan's table in LDS.
 // ---------------------------------------------------------------------------
 
 template <typename R, int P, bool LINE_FAST>
 __device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
-                                            int nlines, int ls, int es, const LinePlan<R>& pl, bool inv) {
+                                            int nlines, int ls, int es, const Cx<R>* __restrict__ twb, bool inv) {
     using C = Cx<R>;
     const int nbf = L / P;
     const int total = nbf * nlines;
-    const int twstep = pl.tws * (L / (Ns * P));
+    const int dv = LINE_FAST ? nlines : nbf;
+    const float rdv = 1.0f / float(dv);
+    const float rns = 1.0f / float(Ns);
     for (int it = threadIdx.x; it < total; it += blockDim.x) {
         int b, j;
-        if (LINE_FAST) { j = it / nlines; b = it - j * nlines; }
-        else { b = it / nbf; j = it - b * nbf; }
-        const int k = j % Ns;
+        if (LINE_FAST) { j = fdiv(it, dv, rdv); b = it - j * dv; }
+        else { b = fdiv(it, dv, rdv); j = it - b * dv; }
+        const int q = fdiv(j, Ns, rns);
+        const int k = j - q * Ns;
         const C* s = src + b * ls;
         C v[P];
 #pragma unroll
         for (int r = 0; r < P; ++r) v[r] = s[(j + r * nbf) * es];
         if (Ns > 1) {
+            const C* w = twb + k * (P - 1) - 1;
 #pragma unroll
-            for (int r = 1; r < P; ++r) v[r] = cmulc(v[r], pl.tw[r * k * twstep], inv);
+            for (int r = 1; r < P; ++r) v[r] = cmulc(v[r], w[r], inv);
         }
         bfly<P>(v, inv);
         C* d = dst + b * ls;
-        const int o = (j - k) * P + k;
+        const int o = q * Ns * P + k;
 #pragma unroll
         for (int r = 0; r < P; ++r) d[(o + r * Ns) * es] = v[r];
     }
@@ -196,27 +216,29 @@ __device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>
 // that good_size can produce: 11, 13, 17, 31, 43, 73, 127, 257).
 template <typename R, bool LINE_FAST>
 __device__ __forceinline__ void stage_generic(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
-                                              int P, int nlines, int ls, int es, const LinePlan<R>& pl, bool inv) {
+                                              int P, int nlines, int ls, int es, const Cx<R>* __restrict__ twb,
+                                              const Cx<R>* __restrict__ wP, bool inv) {
     using C = Cx<R>;
     const int nbf = L / P;
     const int total = L * nlines;
-    const int twstep = pl.tws * (L / (Ns * P));
-    const int wp = pl.T / P;
+    const int dv = LINE_FAST ? nlines : L;
+    const float rdv = 1.0f / float(dv), rns = 1.0f / float(Ns), rp = 1.0f / float(P);
     for (int it = threadIdx.x; it < total; it += blockDim.x) {
         int b, o;
-        if (LINE_FAST) { o = it / nlines; b = it - o * nlines; }
-        else { b = it / L; o = it - b * L; }
-        const int k = o % Ns;
-        const int t = o / Ns;
-        const int rout = t % P;
-        const int j = (t / P) * Ns + k;
+        if (LINE_FAST) { o = fdiv(it, dv, rdv); b = it - o * dv; }
+        else { b = fdiv(it, dv, rdv); o = it - b * dv; }
+        const int t = fdiv(o, Ns, rns);
+        const int k = o - t * Ns;
+        const int jq = fdiv(t, P, rp);
+        const int rout = t - jq * P;
+        const int j = jq * Ns + k;
         const C* s = src + b * ls;
         C acc = cmake<C>(R(0), R(0));
         int m = 0;   // (r * rout) mod P
         for (int r = 0; r < P; ++r) {
             C v = s[(j + r * nbf) * es];
-            if (Ns > 1 && r) v = cmulc(v, pl.tw[r * k * twstep], inv);
-            acc = cadd(acc, cmulc(v, pl.tw[m * wp], inv));
+            if (Ns > 1 && r) v = cmulc(v, twb[k * (P - 1) + r - 1], inv);
+            acc = cadd(acc, cmulc(v, wP[m], inv));
             m += rout;
             if (m >= P) m -= P;
         }
@@ -224,10 +246,11 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* __restrict__ src, Cx<
     }
 }
 
-// Runs all stages; input in buf[0]; returns the index (0/1) of the buffer
-// holding the output.  Ends with a barrier.
+// Runs all stages; input in b0; returns the index (0/1) of the buffer holding
+// the output.  Ends with a barrier.
 template <typename R, bool LINE_FAST>
-__device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, int nlines, int ls, int es, bool inv) {
+__device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, const Cx<R>* tw, int nlines, int ls, int es,
+                         bool inv) {
     Cx<R>* bufs[2] = {b0, b1};
     int cur = 0;
     int Ns = 1;
@@ -235,20 +258,40 @@ __device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, int nlines
         const int P = pl.rad[s];
         const Cx<R>* src = bufs[cur];
         Cx<R>* dst = bufs[cur ^ 1];
+        const Cx<R>* twb = tw + pl.base[s];
         switch (P) {
-            case 2: stage_fixed<R, 2, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            case 3: stage_fixed<R, 3, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            case 4: stage_fixed<R, 4, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            case 5: stage_fixed<R, 5, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            case 7: stage_fixed<R, 7, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            case 8: stage_fixed<R, 8, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, pl, inv); break;
-            default: stage_generic<R, LINE_FAST>(src, dst, pl.L, Ns, P, nlines, ls, es, pl, inv); break;
+            case 2: stage_fixed<R, 2, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            case 3: stage_fixed<R, 3, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            case 4: stage_fixed<R, 4, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            case 5: stage_fixed<R, 5, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            case 7: stage_fixed<R, 7, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            case 8: stage_fixed<R, 8, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
+            default: stage_generic<R, LINE_FAST>(src, dst, pl.L, Ns, P, nlines, ls, es, twb, tw + pl.wp[s], inv); break;
         }
         __syncthreads();
         cur ^= 1;
         Ns *= P;
     }
     return cur;
+}
+
+// Copies n entries global -> LDS with kU loads in flight per thread.
+template <typename C>
+__device__ __forceinline__ void stage_table(C* __restrict__ dst, const C* __restrict__ src, int n) {
+    const int T = blockDim.x;
+    for (int base = threadIdx.x; base < n; base += kU * T) {
+        C v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = base + u * T;
+            if (i < n) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = base + u * T;
+            if (i < n) dst[i] = v[u];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -262,6 +305,8 @@ struct PadSrc {
     const int32_t* map[kMaxRank];   // length fft[e]
     E cst[kMaxRank][2];
     int64_t xstr[kMaxRank];
+    int32_t lo[kMaxRank];           // interior [lo, lo + n) maps to x index p - lo
+    int32_t n[kMaxRank];
 };
 
 // What the kernel side needs to read the scattered, dilated kernel.
@@ -296,13 +341,32 @@ __device__ __forceinline__ E zero_of() {
     }
 }
 
+// Value of the padded row at position p of the last axis.  Interior
+// positions are computed without touching the border map.
 template <typename E>
-__device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, int ax, int32_t c) {
-    if (rr.zero || c == kOutside) return zero_of<E>();
+__device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, int ax, int64_t p) {
+    if (rr.zero) return zero_of<E>();
+    const int64_t rel = p - s.lo[ax];
+    if (rel >= 0 && rel < s.n[ax]) {
+        if (rr.oc_axis >= 0) return rr.ocval;
+        if (rr.unw) return zero_of<E>();
+        return s.x[rr.xoff + rel * s.xstr[ax]];
+    }
+    const int32_t c = s.map[ax][p];
+    if (c == kOutside) return zero_of<E>();
     if (c <= kConstFront) return s.cst[ax][c == kConstFront ? 0 : 1];
     if (rr.oc_axis >= 0) return rr.ocval;
     if (c == kUnwritten || rr.unw) return zero_of<E>();
     return s.x[rr.xoff + int64_t(c) * s.xstr[ax]];
+}
+
+template <typename E>
+__device__ __forceinline__ E ker_value(const KerSrc<E>& s, int64_t koff, int ax, int64_t p) {
+    const int dd = s.d[ax];
+    if (p % dd) return zero_of<E>();
+    const int64_t i = p / dd;
+    if (i >= s.K[ax]) return zero_of<E>();
+    return s.k[koff + (s.rev ? i : s.K[ax] - 1 - i) * s.kstr[ax]];
 }
 
 template <typename R, typename E>
@@ -311,9 +375,42 @@ __device__ __forceinline__ Cx<R> to_cx(E v) {
     else return cmake<Cx<R>>(R(v), R(0));
 }
 
+// Complex sample n of a row for the FFT: the pair (x[2n], x[2n+1]) for the
+// half-length real transform, x[n] otherwise.
+template <typename R, typename E, typename F>
+__device__ __forceinline__ Cx<R> row_sample(int mode, int n, F&& val) {
+    if (mode == 0) {
+        if constexpr (std::is_same<E, Cx<R>>::value) return val(n);
+        else return cmake<Cx<R>>(R(val(2 * n)), R(val(2 * n + 1)));
+    }
+    return to_cx<R>(val(n));
+}
+
+// X[k] of a real line of length F = 2M from the half-length spectrum Z
+// (X = E + W_F^k O with E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2).
+template <typename C>
+__device__ __forceinline__ C r2c_bin(const C* Z, int M, int k, const C* wpost) {
+    using R = decltype(C{}.x);
+    const C zk = Z[k == M ? 0 : k];
+    const C zm = cconj(Z[k == 0 ? 0 : M - k]);
+    const C e = cscale(cadd(zk, zm), R(0.5));
+    const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
+    return cadd(e, cmul(o, wpost[k]));
+}
+
+// Z[k] of the half-length inverse from bins A = X[k] and B = conj X[M-k]:
+// (A + B) + i W_F^-k (A - B); IFFT_M of it is F times the real line.
+template <typename C>
+__device__ __forceinline__ C c2r_pre(C A, C B, int k, const C* wpost) {
+    const C t = cmul(csub(A, B), cconj(wpost[k]));
+    return cadd(cadd(A, B), cmul_i(t));
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: forward rows (last axis) for data (job 0) and kernel (job 1)
 // ---------------------------------------------------------------------------
+
+constexpr int kMaxLpb = 16;
 
 template <typename R, typename E>
 struct RowsFwdArgs {
@@ -334,15 +431,16 @@ struct RowsFwdArgs {
 };
 
 template <typename R, typename E>
-__global__ void __launch_bounds__(kThreads) rows_fwd_kernel(RowsFwdArgs<R, E> a) {
+__global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
-    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + a.lpb * a.ls;
-    __shared__ int64_t s_off[16];
-    __shared__ int64_t s_xoff[16];
-    __shared__ int s_flag[16];     // bit0 zero row, bit1 unwritten, oc axis in bits 8..
-    __shared__ E s_oc[16];
+    __shared__ int64_t s_off[kMaxLpb];
+    __shared__ int64_t s_xoff[kMaxLpb];
+    __shared__ int s_flag[kMaxLpb];     // bit0 zero row, bit1 unwritten, oc axis + 1 in bits 8..
+    __shared__ E s_oc[kMaxLpb];
 
     const int job = blockIdx.x < unsigned(a.nblocks0) ? 0 : 1;
     const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
@@ -380,67 +478,54 @@ __global__ void __launch_bounds__(kThreads) rows_fwd_kernel(RowsFwdArgs<R, E> a)
         s_flag[threadIdx.x] = flag | ((oc + 1) << 8);
         s_oc[threadIdx.x] = ocv;
     }
+    stage_table(tw, a.pl.tw, a.pl.ntw);
     __syncthreads();
 
-    // load
+    // batched load of the rows
     const int L = a.pl.L;
     const int total = nl * L;
-    for (int it = threadIdx.x; it < total; it += kThreads) {
-        const int b = it / L, n = it - b * L;
-        const int flag = s_flag[b];
-        E v0 = zero_of<E>(), v1 = zero_of<E>();
-        if (!(flag & 1)) {
-            if (job == 0) {
-                RowRes<E> rr{s_xoff[b], (flag >> 8) - 1, s_oc[b], (flag & 2) != 0, false};
-                if (a.mode == 0) {
-                    v0 = pad_value(a.pad, rr, ax, a.pad.map[ax][2 * n]);
-                    v1 = pad_value(a.pad, rr, ax, a.pad.map[ax][2 * n + 1]);
-                } else {
-                    v0 = pad_value(a.pad, rr, ax, a.pad.map[ax][n]);
+    const int T = blockDim.x;
+    const float rL = 1.0f / float(L);
+    for (int base = threadIdx.x; base < total; base += kU * T) {
+        C z[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            z[u] = cmake<C>(R(0), R(0));
+            if (it < total) {
+                const int b = fdiv(it, L, rL), n = it - b * L;
+                const int flag = s_flag[b];
+                if (!(flag & 1)) {
+                    if (job == 0) {
+                        const RowRes<E> rr{s_xoff[b], (flag >> 8) - 1, s_oc[b], (flag & 2) != 0, false};
+                        z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return pad_value(a.pad, rr, ax, p); });
+                    } else {
+                        const int64_t koff = s_xoff[b];
+                        z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return ker_value(a.ker, koff, ax, p); });
+                    }
                 }
-            } else {
-                const int dd = a.ker.d[ax];
-                auto kv = [&](int p) -> E {
-                    if (p % dd) return zero_of<E>();
-                    const int i = p / dd;
-                    if (i >= a.ker.K[ax]) return zero_of<E>();
-                    const int ki = a.ker.rev ? i : a.ker.K[ax] - 1 - i;
-                    return a.ker.k[s_xoff[b] + int64_t(ki) * a.ker.kstr[ax]];
-                };
-                if (a.mode == 0) { v0 = kv(2 * n); v1 = kv(2 * n + 1); }
-                else v0 = kv(n);
             }
         }
-        C z;
-        if (a.mode == 0) {
-            if constexpr (std::is_same<E, C>::value) z = v0;   // not reached: complex rows use mode 2
-            else z = cmake<C>(R(v0), R(v1));
-        } else {
-            z = to_cx<R>(v0);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            if (it < total) {
+                const int b = fdiv(it, L, rL), n = it - b * L;
+                b0[b * a.ls + n] = z[u];
+            }
         }
-        b0[b * a.ls + n] = z;
     }
     __syncthreads();
-    const int cur = fft_lines<R, false>(b0, b1, a.pl, nl, a.ls, 1, false);
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, false);
     const C* Z = cur ? b1 : b0;
 
     // store H bins per row
     Cx<R>* out = a.spec[job];
     const int H = a.H;
-    const int M = L;
-    for (int it = threadIdx.x; it < nl * H; it += kThreads) {
-        const int b = it / H, k = it - b * H;
-        C X;
-        if (a.mode == 0) {
-            // X[k] = E[k] + W_F^k O[k], E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2
-            const C zk = Z[b * a.ls + (k == M ? 0 : k)];
-            const C zm = cconj(Z[b * a.ls + (k == 0 ? 0 : M - k)]);
-            const C e = cscale(cadd(zk, zm), R(0.5));
-            const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
-            X = cadd(e, cmul(o, a.pl.tw[k]));
-        } else {
-            X = Z[b * a.ls + k];
-        }
+    const float rH = 1.0f / float(H);
+    for (int it = threadIdx.x; it < nl * H; it += T) {
+        const int b = fdiv(it, H, rH), k = it - b * H;
+        const C X = a.mode == 0 ? r2c_bin(Z + b * a.ls, L, k, tw + a.pl.post) : Z[b * a.ls + k];
         out[s_off[b] + k] = X;
     }
 }
@@ -469,19 +554,20 @@ struct ColsArgs {
     int32_t valid[2];                // rows of axis d holding data (rest are zero)
     AxisSel outsel;                  // rows of axis d to write back
     int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse
-    int32_t ls_pad;                  // unused padding
     LinePlan<R> pl;
 };
 
 template <typename R>
-__global__ void __launch_bounds__(kThreads) cols_kernel(ColsArgs<R> a) {
+__global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
     const int L = a.pl.L;
     const int Ct = a.C;
-    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + L * Ct;
     C* b2 = b1 + L * Ct;
+    const int T = blockDim.x;
 
     const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
     int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
@@ -497,48 +583,59 @@ __global__ void __launch_bounds__(kThreads) cols_kernel(ColsArgs<R> a) {
     const int c0 = tile * Ct;
     const int nc = min(Ct, a.H - c0);
     const int64_t sd = a.sstr[a.d];
+    const float rC = 1.0f / float(Ct);
 
     auto load = [&](const C* src, C* dstbuf, int valid) {
         const int total = L * Ct;
-        for (int it = threadIdx.x; it < total; it += kThreads) {
-            const int t = it / Ct, c = it - t * Ct;
-            C v = cmake<C>(R(0), R(0));
-            if (t < valid && c < nc) v = src[base + int64_t(t) * sd + c0 + c];
-            dstbuf[t * Ct + c] = v;
+        for (int b = threadIdx.x; b < total; b += kU * T) {
+            C v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int it = b + u * T;
+                v[u] = cmake<C>(R(0), R(0));
+                if (it < total) {
+                    const int t = fdiv(it, Ct, rC), c = it - t * Ct;
+                    if (t < valid && c < nc) v[u] = src[base + int64_t(t) * sd + c0 + c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int it = b + u * T;
+                if (it < total) dstbuf[it] = v[u];
+            }
         }
     };
 
+    stage_table(tw, a.pl.tw, a.pl.ntw);
     C* res;
     if (a.mode == 0) {
         load(a.spec[job], b0, a.valid[job]);
         __syncthreads();
-        res = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, false) ? b1 : b0;
+        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, false) ? b1 : b0;
     } else if (a.mode == 1) {
-        // kernel column -> forward; result parked in b2 (or already there)
+        // kernel and data columns loaded together, then transformed in turn
         load(a.spec[1], b1, a.valid[1]);
-        __syncthreads();
-        const int ck = fft_lines<R, true>(b1, b2, a.pl, Ct, 1, Ct, false);
-        C* kres = ck ? b2 : b1;
-        C* kfree = ck ? b1 : b2;
-        // data column -> forward using b0 and the free buffer
         load(a.spec[0], b0, a.valid[0]);
         __syncthreads();
-        const int cd = fft_lines<R, true>(b0, kfree, a.pl, Ct, 1, Ct, false);
+        const int ck = fft_lines<R, true>(b1, b2, a.pl, tw, Ct, 1, Ct, false);
+        C* kres = ck ? b2 : b1;
+        C* kfree = ck ? b1 : b2;
+        const int cd = fft_lines<R, true>(b0, kfree, a.pl, tw, Ct, 1, Ct, false);
         C* dres = cd ? kfree : b0;
         C* dfree = cd ? b0 : kfree;
-        for (int it = threadIdx.x; it < L * Ct; it += kThreads) dres[it] = cmul(dres[it], kres[it]);
+        for (int it = threadIdx.x; it < L * Ct; it += T) dres[it] = cmul(dres[it], kres[it]);
         __syncthreads();
-        res = fft_lines<R, true>(dres, dfree, a.pl, Ct, 1, Ct, true) ? dfree : dres;
+        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true) ? dfree : dres;
     } else {
         load(a.spec[0], b0, L);
         __syncthreads();
-        res = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, true) ? b1 : b0;
+        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, true) ? b1 : b0;
     }
     // store selected rows
     C* dst = a.spec[job];
     const int nrow = int(a.outsel.count);
-    for (int it = threadIdx.x; it < nrow * Ct; it += kThreads) {
-        const int i = it / Ct, c = it - i * Ct;
+    for (int it = threadIdx.x; it < nrow * Ct; it += T) {
+        const int i = fdiv(it, Ct, rC), c = it - i * Ct;
         if (c >= nc) continue;
         const int t = int(a.outsel.start + int64_t(i) * a.outsel.step);
         dst[base + int64_t(t) * sd + c0 + c] = res[t * Ct + c];
@@ -565,13 +662,28 @@ struct RowsInvArgs {
 };
 
 template <typename R, typename E>
-__global__ void __launch_bounds__(kThreads) rows_inv_kernel(RowsInvArgs<R, E> a) {
+__device__ __forceinline__ E store_value(int mode, const Cx<R>* Z, int64_t p, R scale) {
+    if constexpr (std::is_same<E, Cx<R>>::value) {
+        return cscale(Z[p], scale);
+    } else {
+        if (mode == 0) {
+            const Cx<R> zz = Z[p >> 1];
+            return E(((p & 1) ? zz.y : zz.x) * scale);
+        }
+        return E(Z[p].x * scale);
+    }
+}
+
+template <typename R, typename E>
+__global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
-    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + a.lpb * a.ls;
-    __shared__ int64_t s_off[16];
-    __shared__ int64_t s_yoff[16];
+    __shared__ int64_t s_off[kMaxLpb];
+    __shared__ int64_t s_yoff[kMaxLpb];
+    const int T = blockDim.x;
     const int64_t line0 = int64_t(blockIdx.x) * a.lpb;
     const int nl = int(imin64(a.lpb, a.nlines - line0));
     if (threadIdx.x < nl) {
@@ -586,53 +698,61 @@ __global__ void __launch_bounds__(kThreads) rows_inv_kernel(RowsInvArgs<R, E> a)
         s_off[threadIdx.x] = soff;
         s_yoff[threadIdx.x] = yoff;
     }
+    stage_table(tw, a.pl.tw, a.pl.ntw);
     __syncthreads();
     const int L = a.pl.L;
     const int M = L;
-    for (int it = threadIdx.x; it < nl * L; it += kThreads) {
-        const int b = it / L, k = it - b * L;
-        const C* X = a.spec + s_off[b];
-        C z;
-        if (a.mode == 0) {
-            // Z[k] = (A + B) + i W_F^-k (A - B), A = X[k], B = conj X[M-k];
-            // Im X[0] and Im X[M] are ignored as a c2r transform does
-            C A = X[k], B = cconj(X[M - k]);
-            if (k == 0) { A.y = 0; B = cmake<C>(X[M].x, R(0)); }
-            const C s = cadd(A, B);
-            const C t = cmul(csub(A, B), cconj(a.pl.tw[k]));
-            z = cadd(s, cmul_i(t));
-        } else if (a.mode == 1) {
-            // odd F: Hermitian completion of the half spectrum
-            if (k == 0) z = cmake<C>(X[0].x, R(0));
-            else if (k < a.H) z = X[k];
-            else z = cconj(X[L - k]);
-        } else {
-            z = X[k];
+    const float rL = 1.0f / float(L);
+    const int total = nl * L;
+    for (int base = threadIdx.x; base < total; base += kU * T) {
+        C z[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            z[u] = cmake<C>(R(0), R(0));
+            if (it < total) {
+                const int b = fdiv(it, L, rL), k = it - b * L;
+                const C* X = a.spec + s_off[b];
+                if (a.mode == 0) {
+                    // Im X[0] and Im X[M] are ignored as a c2r transform does
+                    C A = X[k], B = cconj(X[M - k]);
+                    if (k == 0) { A.y = 0; B.y = 0; }
+                    z[u] = c2r_pre(A, B, k, tw + a.pl.post);
+                } else if (a.mode == 1) {
+                    // odd F: Hermitian completion of the half spectrum
+                    if (k == 0) z[u] = cmake<C>(X[0].x, R(0));
+                    else if (k < a.H) z[u] = X[k];
+                    else z[u] = cconj(X[L - k]);
+                } else {
+                    z[u] = X[k];
+                }
+            }
         }
-        b0[b * a.ls + k] = z;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            if (it < total) {
+                const int b = fdiv(it, L, rL), k = it - b * L;
+                b0[b * a.ls + k] = z[u];
+            }
+        }
     }
     __syncthreads();
-    const int cur = fft_lines<R, false>(b0, b1, a.pl, nl, a.ls, 1, true);
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, true);
     const C* Z = cur ? b1 : b0;
     const int nout_last = int(a.lastsel.count);
-    for (int it = threadIdx.x; it < nl * nout_last; it += kThreads) {
-        const int b = it / nout_last, i = it - b * nout_last;
-        const int p = int(a.lastsel.start + int64_t(i) * a.lastsel.step);
-        E v;
-        if constexpr (std::is_same<E, C>::value) {
-            v = cscale(Z[b * a.ls + p], a.scale);
-        } else if (a.mode == 0) {
-            const C zz = Z[b * a.ls + (p >> 1)];
-            v = E(((p & 1) ? zz.y : zz.x) * a.scale);
-        } else {
-            v = E(Z[b * a.ls + p].x * a.scale);
-        }
-        a.y[s_yoff[b] + i] = v;
+    const float rn = 1.0f / float(nout_last);
+    for (int it = threadIdx.x; it < nl * nout_last; it += T) {
+        const int b = fdiv(it, nout_last, rn), i = it - b * nout_last;
+        const int64_t p = a.lastsel.start + int64_t(i) * a.lastsel.step;
+        a.y[s_yoff[b] + i] = store_value<R, E>(a.mode, Z + b * a.ls, p, a.scale);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Rank 1: kernel and data transforms, multiply, inverse and crop in one block
+// (the kernel spectrum is transformed in the block, or read from a previous
+// pass when three line buffers do not fit LDS)
 // ---------------------------------------------------------------------------
 
 template <typename R, typename E>
@@ -648,110 +768,83 @@ struct Fused1dArgs {
 };
 
 template <typename R, typename E>
-__global__ void __launch_bounds__(kThreads) fused1d_kernel(Fused1dArgs<R, E> a) {
+__global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
     const int L = a.pl.L;
-    C* buf[3] = {reinterpret_cast<C*>(smem_raw), reinterpret_cast<C*>(smem_raw) + L,
-                 reinterpret_cast<C*>(smem_raw) + 2 * L};
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* buf[3] = {tw + a.pl.ntw, tw + a.pl.ntw + L, tw + a.pl.ntw + 2 * L};
+    const int T = blockDim.x;
     const RowRes<E> rr{0, -1, zero_of<E>(), false, false};
-    auto kv = [&](int p) -> E {
-        const int dd = a.ker.d[0];
-        if (p % dd) return zero_of<E>();
-        const int i = p / dd;
-        if (i >= a.ker.K[0]) return zero_of<E>();
-        const int ki = a.ker.rev ? i : a.ker.K[0] - 1 - i;
-        return a.ker.k[int64_t(ki) * a.ker.kstr[0]];
-    };
     auto load = [&](C* dst, bool kernel) {
-        for (int n = threadIdx.x; n < L; n += kThreads) {
-            C z;
-            if (a.mode == 0) {
-                E v0, v1;
-                if (kernel) { v0 = kv(2 * n); v1 = kv(2 * n + 1); }
-                else { v0 = pad_value(a.pad, rr, 0, a.pad.map[0][2 * n]); v1 = pad_value(a.pad, rr, 0, a.pad.map[0][2 * n + 1]); }
-                if constexpr (std::is_same<E, C>::value) z = v0;
-                else z = cmake<C>(R(v0), R(v1));
-            } else {
-                E v0 = kernel ? kv(n) : pad_value(a.pad, rr, 0, a.pad.map[0][n]);
-                z = to_cx<R>(v0);
+        for (int base = threadIdx.x; base < L; base += kU * T) {
+            C z[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int n = base + u * T;
+                z[u] = cmake<C>(R(0), R(0));
+                if (n < L) {
+                    if (kernel) z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return ker_value(a.ker, 0, 0, p); });
+                    else z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return pad_value(a.pad, rr, 0, p); });
+                }
             }
-            dst[n] = z;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int n = base + u * T;
+                if (n < L) dst[n] = z[u];
+            }
         }
     };
-    // kernel spectrum: transformed here (three line buffers) or read from
-    // global memory (two line buffers)
+    stage_table(tw, a.pl.tw, a.pl.ntw);
     C* K = nullptr;
     C* f0 = buf[0];
     C* f1 = buf[1];
     if (!a.ks) {
-        load(buf[0], true);
+        load(buf[2], true);
+        load(buf[0], false);
         __syncthreads();
-        const int ck = fft_lines<R, false>(buf[0], buf[1], a.pl, 1, L, 1, false);
-        K = buf[ck];
-        f0 = buf[ck ^ 1];
-        f1 = buf[2];
+        const int ck = fft_lines<R, false>(buf[2], buf[1], a.pl, tw, 1, L, 1, false);
+        K = ck ? buf[1] : buf[2];
+        f1 = ck ? buf[2] : buf[1];
+    } else {
+        load(buf[0], false);
+        __syncthreads();
     }
-    load(f0, false);
-    __syncthreads();
-    const int cd = fft_lines<R, false>(f0, f1, a.pl, 1, L, 1, false);
+    const int cd = fft_lines<R, false>(f0, f1, a.pl, tw, 1, L, 1, false);
     C* D = cd ? f1 : f0;
     C* W = cd ? f0 : f1;   // free buffer
+    const C* wpost = tw + a.pl.post;
     if (a.mode == 0) {
         // bins k and M-k together: forward post-process both spectra,
         // multiply, then the c2r pre-process of the product
         const int M = L;
-        for (int k = threadIdx.x; k <= M / 2; k += kThreads) {
+        for (int k = threadIdx.x; k <= M / 2; k += T) {
             const int km = M - k;
-            auto bin = [&](const C* Z, int kk) {
-                const C zk = Z[kk == M ? 0 : kk];
-                const C zm = cconj(Z[kk == 0 ? 0 : M - kk]);
-                const C e = cscale(cadd(zk, zm), R(0.5));
-                const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
-                return cadd(e, cmul(o, a.pl.tw[kk]));
-            };
-            C Pk = cmul(bin(D, k), a.ks ? a.ks[k] : bin(K, k));
-            C Pm = cmul(bin(D, km), a.ks ? a.ks[km] : bin(K, km));
+            C Pk = cmul(r2c_bin(D, M, k, wpost), a.ks ? a.ks[k] : r2c_bin(K, M, k, wpost));
+            C Pm = cmul(r2c_bin(D, M, km, wpost), a.ks ? a.ks[km] : r2c_bin(K, M, km, wpost));
             if (k == 0) { Pk.y = 0; Pm.y = 0; }   // Im of DC and Nyquist ignored by c2r
-            {
-                const C A = Pk, B = cconj(Pm);
-                const C t = cmul(csub(A, B), cconj(a.pl.tw[k]));
-                W[k == M ? 0 : k] = cadd(cadd(A, B), cmul_i(t));
-            }
-            if (km != k && km < M) {
-                const C A = Pm, B = cconj(Pk);
-                const C t = cmul(csub(A, B), cconj(a.pl.tw[km]));
-                W[km] = cadd(cadd(A, B), cmul_i(t));
-            }
+            W[k == M ? 0 : k] = c2r_pre(Pk, cconj(Pm), k, wpost);
+            if (km != k && km < M) W[km] = c2r_pre(Pm, cconj(Pk), km, wpost);
         }
     } else if (a.mode == 1) {
         // odd F: the product of two Hermitian spectra, used whole
-        for (int k = threadIdx.x; k < L; k += kThreads) {
+        for (int k = threadIdx.x; k < L; k += T) {
             // the stored kernel spectrum holds bins 0..F/2; the rest by symmetry
             const C kk = a.ks ? (k < a.H ? a.ks[k] : cconj(a.ks[L - k])) : K[k];
             W[k] = cmul(D[k], kk);
         }
         if (threadIdx.x == 0) W[0].y = 0;
     } else {
-        for (int k = threadIdx.x; k < L; k += kThreads) W[k] = cmul(D[k], a.ks ? a.ks[k] : K[k]);
+        for (int k = threadIdx.x; k < L; k += T) W[k] = cmul(D[k], a.ks ? a.ks[k] : K[k]);
     }
     __syncthreads();
     C* other = (W == f0) ? f1 : f0;
-    const int ci = fft_lines<R, false>(W, other, a.pl, 1, L, 1, true);
+    const int ci = fft_lines<R, false>(W, other, a.pl, tw, 1, L, 1, true);
     const C* Z = ci ? other : W;
     const int nout = int(a.lastsel.count);
-    for (int i = threadIdx.x; i < nout; i += kThreads) {
-        const int p = int(a.lastsel.start + int64_t(i) * a.lastsel.step);
-        E v;
-        if constexpr (std::is_same<E, C>::value) {
-            v = cscale(Z[p], a.scale);
-        } else if (a.mode == 0) {
-            const C zz = Z[p >> 1];
-            v = E(((p & 1) ? zz.y : zz.x) * a.scale);
-        } else {
-            v = E(Z[p].x * a.scale);
-        }
-        a.y[i] = v;
+    for (int i = threadIdx.x; i < nout; i += T) {
+        const int64_t p = a.lastsel.start + int64_t(i) * a.lastsel.step;
+        a.y[i] = store_value<R, E>(a.mode, Z, p, a.scale);
     }
 }
 
@@ -787,12 +880,14 @@ struct BigColArgs {
 };
 
 template <typename R, typename E>
-__global__ void __launch_bounds__(kThreads) bigcol_kernel(BigColArgs<R, E> a) {
+__global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
     const int L1 = a.pl.L, L2 = a.L2, Ct = a.Ct;
-    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + L1 * Ct;
+    const int T = blockDim.x;
     const int job = (!a.inv && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
     const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
     const int tile = int(blk % a.ntiles);
@@ -829,48 +924,50 @@ __global__ void __launch_bounds__(kThreads) bigcol_kernel(BigColArgs<R, E> a) {
     const int c0 = tile * Ct;
     const int nc = min(Ct, L2 - c0);
     const int ax = a.nout;
-    for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
-        const int n1 = it / Ct, c = it - n1 * Ct;
-        C v = cmake<C>(R(0), R(0));
-        if (c < nc) {
-            const int n2 = c0 + c;
-            if (!a.inv) {
-                const int64_t n = int64_t(n1) * L2 + n2;
-                if (!zero) {
-                    E e;
-                    if (job == 0) {
-                        RowRes<E> rr{xoff, oc, ocv, unw, false};
-                        e = pad_value(a.pad, rr, ax, a.pad.map[ax][n]);
+    const float rC = 1.0f / float(Ct);
+    stage_table(tw, a.pl.tw, a.pl.ntw);
+    const RowRes<E> rr{xoff, oc, ocv, unw, zero};
+    const int total = L1 * Ct;
+    for (int base = threadIdx.x; base < total; base += kU * T) {
+        C z[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            z[u] = cmake<C>(R(0), R(0));
+            if (it < total) {
+                const int n1 = fdiv(it, Ct, rC), c = it - n1 * Ct;
+                if (c < nc) {
+                    const int n2 = c0 + c;
+                    if (!a.inv) {
+                        const int64_t n = int64_t(n1) * L2 + n2;
+                        if (job == 0) z[u] = to_cx<R>(pad_value(a.pad, rr, ax, n));
+                        else if (!zero) z[u] = to_cx<R>(ker_value(a.ker, xoff, ax, n));
                     } else {
-                        const int dd = a.ker.d[ax];
-                        e = zero_of<E>();
-                        if (n % dd == 0 && n / dd < a.ker.K[ax]) {
-                            const int64_t i = n / dd;
-                            e = a.ker.k[xoff + (a.ker.rev ? i : a.ker.K[ax] - 1 - i) * a.ker.kstr[ax]];
-                        }
+                        z[u] = cmulc(a.spec[0][soff + int64_t(n1) * L2 + n2], a.twL[int64_t(n1) * n2], true);
                     }
-                    v = to_cx<R>(e);
                 }
-            } else {
-                v = cmulc(a.spec[0][soff + int64_t(n1) * L2 + n2], a.twL[int64_t(n1) * n2], true);
             }
         }
-        b0[n1 * Ct + c] = v;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            if (it < total) b0[it] = z[u];
+        }
     }
     __syncthreads();
-    const int cur = fft_lines<R, true>(b0, b1, a.pl, Ct, 1, Ct, a.inv != 0);
+    const int cur = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, a.inv != 0);
     const C* Z = cur ? b1 : b0;
     if (!a.inv) {
         C* out = a.spec[job];
-        for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
-            const int k1 = it / Ct, c = it - k1 * Ct;
+        for (int it = threadIdx.x; it < total; it += T) {
+            const int k1 = fdiv(it, Ct, rC), c = it - k1 * Ct;
             if (c >= nc) continue;
             const int n2 = c0 + c;
-            out[soff + int64_t(k1) * L2 + n2] = cmul(Z[k1 * Ct + c], a.twL[int64_t(k1) * n2]);
+            out[soff + int64_t(k1) * L2 + n2] = cmul(Z[it], a.twL[int64_t(k1) * n2]);
         }
     } else {
-        for (int it = threadIdx.x; it < L1 * Ct; it += kThreads) {
-            const int n1 = it / Ct, c = it - n1 * Ct;
+        for (int it = threadIdx.x; it < total; it += T) {
+            const int n1 = fdiv(it, Ct, rC), c = it - n1 * Ct;
             if (c >= nc) continue;
             const int64_t n = int64_t(n1) * L2 + c0 + c;
             const int64_t rel = n - a.lastsel.start;
@@ -878,8 +975,8 @@ __global__ void __launch_bounds__(kThreads) bigcol_kernel(BigColArgs<R, E> a) {
             const int64_t i = rel / a.lastsel.step;
             if (i >= a.lastsel.count) continue;
             E v;
-            if constexpr (std::is_same<E, C>::value) v = cscale(Z[n1 * Ct + c], a.scale);
-            else v = E(Z[n1 * Ct + c].x * a.scale);
+            if constexpr (std::is_same<E, C>::value) v = cscale(Z[it], a.scale);
+            else v = E(Z[it].x * a.scale);
             a.y[yoff + i] = v;
         }
     }
@@ -897,13 +994,15 @@ struct BigRowArgs {
 };
 
 template <typename R>
-__global__ void __launch_bounds__(kThreads) bigrow_kernel(BigRowArgs<R> a) {
+__global__ void __launch_bounds__(kMaxThreads) bigrow_kernel(BigRowArgs<R> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
     const int L2 = a.L2;
-    C* b0 = reinterpret_cast<C*>(smem_raw);
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + L2;
     C* b2 = b1 + L2;
+    const int T = blockDim.x;
     const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
     const int64_t row = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
     const int64_t k1 = row % a.L1;
@@ -915,27 +1014,27 @@ __global__ void __launch_bounds__(kThreads) bigrow_kernel(BigRowArgs<R> a) {
         rem /= s.count;
         soff += (s.start + i * s.step) * a.sstr[e];
     }
+    stage_table(tw, a.pl.tw, a.pl.ntw);
     C* res;
     if (a.mode == 1) {
-        for (int n = threadIdx.x; n < L2; n += kThreads) b1[n] = a.spec[1][soff + n];
+        stage_table(b1, a.spec[1] + soff, L2);
+        stage_table(b0, a.spec[0] + soff, L2);
         __syncthreads();
-        const int ck = fft_lines<R, false>(b1, b2, a.pl, 1, L2, 1, false);
+        const int ck = fft_lines<R, false>(b1, b2, a.pl, tw, 1, L2, 1, false);
         C* K = ck ? b2 : b1;
         C* f1 = ck ? b1 : b2;
-        for (int n = threadIdx.x; n < L2; n += kThreads) b0[n] = a.spec[0][soff + n];
-        __syncthreads();
-        const int cd = fft_lines<R, false>(b0, f1, a.pl, 1, L2, 1, false);
+        const int cd = fft_lines<R, false>(b0, f1, a.pl, tw, 1, L2, 1, false);
         C* D = cd ? f1 : b0;
         C* Dfree = cd ? b0 : f1;
-        for (int n = threadIdx.x; n < L2; n += kThreads) D[n] = cmul(D[n], K[n]);
+        for (int n = threadIdx.x; n < L2; n += T) D[n] = cmul(D[n], K[n]);
         __syncthreads();
-        res = fft_lines<R, false>(D, Dfree, a.pl, 1, L2, 1, true) ? Dfree : D;
+        res = fft_lines<R, false>(D, Dfree, a.pl, tw, 1, L2, 1, true) ? Dfree : D;
     } else {
-        for (int n = threadIdx.x; n < L2; n += kThreads) b0[n] = a.spec[job][soff + n];
+        stage_table(b0, a.spec[job] + soff, L2);
         __syncthreads();
-        res = fft_lines<R, false>(b0, b1, a.pl, 1, L2, 1, a.mode == 2) ? b1 : b0;
+        res = fft_lines<R, false>(b0, b1, a.pl, tw, 1, L2, 1, a.mode == 2) ? b1 : b0;
     }
-    for (int n = threadIdx.x; n < L2; n += kThreads) a.spec[job][soff + n] = res[n];
+    for (int n = threadIdx.x; n < L2; n += T) a.spec[job][soff + n] = res[n];
 }
 
 // ---------------------------------------------------------------------------
@@ -945,8 +1044,7 @@ __global__ void __launch_bounds__(kThreads) bigrow_kernel(BigRowArgs<R> a) {
 std::vector<int> factor_radices(int L) {
     std::vector<int> r;
     int n = L;
-    while (n % 8 == 0 && n > 8) { r.push_back(8); n /= 8; }
-    if (n % 8 == 0) { r.push_back(8); n /= 8; }
+    while (n % 8 == 0) { r.push_back(8); n /= 8; }
     while (n % 4 == 0) { r.push_back(4); n /= 4; }
     while (n % 2 == 0) { r.push_back(2); n /= 2; }
     for (int p : {3, 5, 7}) while (n % p == 0) { r.push_back(p); n /= p; }
@@ -957,18 +1055,64 @@ std::vector<int> factor_radices(int L) {
     return r;
 }
 
+bool fixed_radix(int p) { return p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8; }
+
+// Entries of the compact twiddle table make_lineplan builds for (L, post_F).
+int64_t ntw_of(int L, int post_F) {
+    int64_t n = 0, Ns = 1;
+    for (int P : factor_radices(L)) {
+        n += Ns * (P - 1);
+        if (!fixed_radix(P)) n += P;
+        Ns *= P;
+    }
+    if (post_F > 0) n += L + 1;
+    return n;
+}
+
+// Builds the compact FP64 twiddle table of a line plan and caches it on the
+// device.  post_F > 0 appends W_F^k, k = 0..L (real-even rows, F = 2L).
 template <typename R>
-bool make_lineplan(LinePlan<R>* pl, int L, int tws, LaunchContext& ctx) {
+bool make_lineplan(LinePlan<R>* pl, int L, int post_F, LaunchContext& ctx) {
+    const std::string key = "lineplan:" + std::to_string(L) + ":" + std::to_string(post_F) + ":" +
+                            std::to_string(sizeof(R));
+    std::string& m = ctx.memo(key);
+    if (m.size() == sizeof(*pl)) {
+        std::memcpy(pl, m.data(), sizeof(*pl));
+        return true;
+    }
     std::memset(pl, 0, sizeof(*pl));
     pl->L = L;
-    pl->tws = tws;
-    pl->T = L * tws;
     auto r = factor_radices(L);
     if (int(r.size()) > kMaxStages) return false;
     pl->nst = int(r.size());
-    for (size_t i = 0; i < r.size(); ++i) pl->rad[i] = r[i];
-    pl->tw = static_cast<const Cx<R>*>(ctx.twiddles(pl->T, sizeof(R) == 8));
-    return pl->tw != nullptr;
+    std::vector<double> tab;   // interleaved re, im
+    auto push = [&](double ang) {
+        tab.push_back(std::cos(ang));
+        tab.push_back(std::sin(ang));
+    };
+    int Ns = 1;
+    for (size_t s = 0; s < r.size(); ++s) {
+        const int P = r[s];
+        pl->rad[s] = P;
+        pl->base[s] = int(tab.size() / 2);
+        for (int k = 0; k < Ns; ++k)
+            for (int q = 1; q < P; ++q) push(-2.0 * M_PI * double(int64_t(q) * k) / double(int64_t(Ns) * P));
+        pl->wp[s] = int(tab.size() / 2);
+        if (!fixed_radix(P))
+            for (int m = 0; m < P; ++m) push(-2.0 * M_PI * double(m) / double(P));
+        Ns *= P;
+    }
+    pl->post = -1;
+    if (post_F > 0) {
+        pl->post = int(tab.size() / 2);
+        for (int k = 0; k <= L; ++k) push(-2.0 * M_PI * double(k) / double(post_F));
+    }
+    pl->ntw = int(tab.size() / 2);
+    std::vector<R> dev(tab.begin(), tab.end());
+    pl->tw = static_cast<const Cx<R>*>(ctx.constant_table(key, dev.data(), dev.size() * sizeof(R)));
+    if (!pl->tw) return false;
+    m.assign(reinterpret_cast<const char*>(pl), sizeof(*pl));
+    return true;
 }
 
 template <typename E>
@@ -987,6 +1131,12 @@ ndc_status check_launch(LaunchContext& ctx, const char* what) {
     return NDC_OK;
 }
 
+// Threads per block: about two complex points per thread, 64..1024.
+int threads_for(int64_t points) {
+    int64_t t = (points / 2 + 63) / 64 * 64;
+    return int(std::max<int64_t>(64, std::min<int64_t>(kMaxThreads, t)));
+}
+
 template <typename R, typename E>
 ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, LaunchContext& ctx) {
     using C = Cx<R>;
@@ -997,22 +1147,25 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     const size_t es = sizeof(C);
     const size_t lds = ctx.max_lds();
     int mode;             // 0 real even, 1 real odd, 2 complex
-    int L, tws;
-    if (kComplex) { mode = 2; L = Fl; tws = 1; }
-    else if (Fl % 2 == 0) { mode = 0; L = Fl / 2; tws = 2; }
-    else { mode = 1; L = Fl; tws = 1; }
-    // Lines that do not fit two LDS buffers take the four-step path, as
-    // complex transforms of the full length (all Fl bins kept).
-    const bool big = size_t(2) * L * es > lds;
+    int L;
+    if (kComplex) { mode = 2; L = Fl; }
+    else if (Fl % 2 == 0) { mode = 0; L = Fl / 2; }
+    else { mode = 1; L = Fl; }
+    // LDS of a row plan: the compact table (about L entries, 2L for real-even)
+    // plus two line buffers.  Lines that do not fit take the four-step path,
+    // as complex transforms of the full length (all Fl bins kept).
+    const size_t row_tw = size_t(ntw_of(L, mode == 0 ? Fl : 0));
+    const bool big = (row_tw + size_t(2) * L) * es > lds;
     const int H = big ? Fl : (kComplex ? Fl : Fl / 2 + 1);
     int L1 = 0, L2 = 0, Ctb = 0;
     if (big) {
         for (int d2 = Fl; d2 >= 2; --d2) {
-            if (Fl % d2 || size_t(3) * d2 * es > lds) continue;
+            if (Fl % d2 || (size_t(ntw_of(d2, 0)) + size_t(3) * d2) * es > lds) continue;
             const int d1 = Fl / d2;
+            const size_t t1 = size_t(ntw_of(d1, 0));
             int ct = 16;
-            while (ct > 1 && size_t(2) * d1 * ct * es > lds) ct /= 2;
-            if (size_t(2) * d1 * ct * es > lds) continue;
+            while (ct > 1 && (t1 + size_t(2) * d1 * ct) * es > lds) ct /= 2;
+            if ((t1 + size_t(2) * d1 * ct) * es > lds) continue;
             L1 = d1;
             L2 = d2;
             Ctb = ct;
@@ -1024,7 +1177,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         }
     }
     LinePlan<R> rowpl;
-    if (!big && !make_lineplan<R>(&rowpl, L, tws, ctx)) {
+    if (!big && !make_lineplan<R>(&rowpl, L, mode == 0 ? Fl : 0, ctx)) {
         ctx.err = "twiddle table allocation failed";
         return NDC_ERR_OUT_OF_MEMORY;
     }
@@ -1044,6 +1197,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         pad.cst[i][0] = host_value<E>(p.side[i][0].value);
         pad.cst[i][1] = host_value<E>(p.side[i][1].value);
         pad.xstr[i] = p.x_str[i];
+        pad.lo[i] = int32_t(p.pad[i][0]);
+        pad.n[i] = int32_t(p.x_shape[i]);
         ker.kstr[i] = p.k_str[i];
         ker.K[i] = int32_t(p.k_shape[i]);
         ker.d[i] = int32_t(p.dil[i]);
@@ -1055,7 +1210,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     const double esz = double(sizeof(E));
 
     // rank 1, line fits three buffers: one fused launch
-    if (N == 1 && !big && size_t(3) * L * es <= lds) {
+    if (N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * L) * es <= lds) {
         Fused1dArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
         a.pad = pad;
@@ -1067,10 +1222,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = size_t(3) * L * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(3) * L) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.k_numel + p.y_numel), 15.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "fused rank-1 FFT convolution");
     }
@@ -1089,6 +1244,12 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         return NDC_ERR_OUT_OF_MEMORY;
     }
     const AxisSel none{0, 1, 1};
+
+    auto rows_lpb = [&]() {
+        int lpb = int(std::max<int64_t>(1, std::min<int64_t>(kMaxLpb, 512 / std::max(1, L))));
+        while (lpb > 1 && (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es > lds) lpb /= 2;
+        return lpb;
+    };
 
     auto rows_fwd = [&](bool data_job) -> ndc_status {
         RowsFwdArgs<R, E> a;
@@ -1113,10 +1274,9 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.H = H;
         a.mode = mode;
         a.ls = L;
-        int lpb = int(std::max<int64_t>(1, std::min<int64_t>(16, (2 * kThreads) / std::max(1, L))));
-        while (lpb > 1 && size_t(2) * lpb * L * es > 64 * 1024) lpb /= 2;
+        const int lpb = rows_lpb();
         a.lpb = lpb;
-        const size_t smem = size_t(2) * lpb * L * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es;
         a.pl = rowpl;
         const int64_t nb0 = (a.nlines[0] + lpb - 1) / lpb;
         const int64_t nb1 = (a.nlines[1] + lpb - 1) / lpb;
@@ -1125,7 +1285,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
                              double(es) * double(a.nlines[0] + a.nlines[1]) * H;
         ctx.kbegin("fft_rows_fwd", bytes, 5.0 * double(a.nlines[0] + a.nlines[1]) * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_fwd_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((rows_fwd_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((rows_fwd_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(lpb) * L)),
+                           smem, ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "rows forward");
     };
@@ -1146,10 +1307,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = size_t(2) * L * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * L) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.y_numel) + double(es) * H, 10.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "fused rank-1 FFT convolution");
     }
@@ -1158,7 +1319,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     LinePlan<R> pl1, pl2;
     const Cx<R>* twL = nullptr;
     if (big) {
-        if (!make_lineplan<R>(&pl1, L1, 1, ctx) || !make_lineplan<R>(&pl2, L2, 1, ctx) ||
+        if (!make_lineplan<R>(&pl1, L1, 0, ctx) || !make_lineplan<R>(&pl2, L2, 0, ctx) ||
             !(twL = static_cast<const Cx<R>*>(ctx.twiddles(Fl, sizeof(R) == 8)))) {
             ctx.err = "twiddle table allocation failed";
             return NDC_ERR_OUT_OF_MEMORY;
@@ -1202,13 +1363,14 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.pl = pl1;
         const int64_t nb0 = a.nlines[0] * a.ntiles, nb1 = a.nlines[1] * a.ntiles;
         a.nblocks0 = int32_t(nb0);
-        const size_t smem = size_t(2) * L1 * Ctb * es;
+        const size_t smem = (size_t(pl1.ntw) + size_t(2) * L1 * Ctb) * es;
         const double lines = double(a.nlines[0] + a.nlines[1]);
         const double bytes = inv ? double(es) * lines * Fl + esz * double(p.y_numel)
                                  : esz * double(p.x_numel + p.k_numel) + double(es) * lines * Fl;
         ctx.kbegin(inv ? "fft_bigcol_inv" : "fft_bigcol_fwd", bytes, 5.0 * lines * Fl * std::log2(double(L1)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigcol_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((bigcol_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((bigcol_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(L1) * Ctb)),
+                           smem, ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "long-axis column pass");
     };
@@ -1237,13 +1399,13 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
             if (rmode == 0) a.nrows[1] *= a.sel[1][e].count;
         }
         a.nblocks0 = int32_t(a.nrows[0]);
-        const size_t smem = size_t(rmode == 1 ? 3 : 2) * L2 * es;
+        const size_t smem = (size_t(pl2.ntw) + size_t(rmode == 1 ? 3 : 2) * L2) * es;
         const double rows = double(a.nrows[0] + a.nrows[1]);
         const double bytes = double(es) * L2 * (rmode == 1 ? 3.0 * a.nrows[0] : 2.0 * rows);
         static const char* names[3] = {"fft_bigrow_fwd", "fft_bigrow_fused", "fft_bigrow_inv"};
         ctx.kbegin(names[rmode], bytes, 5.0 * rows * (rmode == 1 ? 3 : 1) * L2 * std::log2(double(L2)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigrow_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((bigrow_kernel<R>), dim3(unsigned(a.nrows[0] + a.nrows[1])), dim3(kThreads), smem,
+        hipLaunchKernelGGL((bigrow_kernel<R>), dim3(unsigned(a.nrows[0] + a.nrows[1])), dim3(threads_for(L2)), smem,
                            ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "long-axis row pass");
@@ -1275,11 +1437,11 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.d = d;
         a.H = H;
         a.mode = cmode;
-        if (!make_lineplan<R>(&a.pl, int(p.fft[d]), 1, ctx)) {
+        const int Ld = int(p.fft[d]);
+        if (!make_lineplan<R>(&a.pl, Ld, 0, ctx)) {
             ctx.err = "twiddle table allocation failed";
             return NDC_ERR_OUT_OF_MEMORY;
         }
-        const int Ld = int(p.fft[d]);
         int no = 0;
         for (int e = 0; e < last; ++e) {
             if (e == d) continue;
@@ -1305,9 +1467,12 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const int nbuf = cmode == 1 ? 3 : 2;
         // tile width: enough lines for coalescing, enough blocks to fill the chip
         int Ct = 16;
-        while (Ct > 1 && size_t(nbuf) * Ld * Ct * es > 64 * 1024) Ct /= 2;
-        while (Ct > 4 && a.nlines_outer[0] * ((H + Ct - 1) / Ct) < 2 * ctx.cu_count()) Ct /= 2;
-        const size_t smem = size_t(nbuf) * Ld * Ct * es;
+        auto smem_of = [&](int ct) { return (size_t(a.pl.ntw) + size_t(nbuf) * Ld * ct) * es; };
+        while (Ct > 1 && smem_of(Ct) > std::min<size_t>(lds, 96 * 1024)) Ct /= 2;
+        // 8 columns = 64-byte segments per row (f32): below that, partial-line
+        // reads and writes cost more than the extra blocks gain
+        while (Ct > 8 && a.nlines_outer[0] * ((H + Ct - 1) / Ct) < 2 * ctx.cu_count()) Ct /= 2;
+        const size_t smem = smem_of(Ct);
         if (smem > lds) {
             ctx.err = "FFT length of a column axis exceeds the LDS-resident line size";
             return NDC_ERR_UNSUPPORTED;
@@ -1327,7 +1492,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         static const char* names[3] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv"};
         ctx.kbegin(names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&cols_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((cols_kernel<R>), dim3(unsigned(nb0 + nb1)), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((cols_kernel<R>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(Ld) * Ct / 2)), smem,
+                           ctx.stream(), a);
         ctx.kend();
         return check_launch(ctx, "column pass");
     };
@@ -1374,18 +1540,18 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.H = H;
         a.mode = mode;
         a.ls = L;
-        int lpb = int(std::max<int64_t>(1, std::min<int64_t>(16, (2 * kThreads) / std::max(1, L))));
-        while (lpb > 1 && size_t(2) * lpb * L * es > 64 * 1024) lpb /= 2;
+        const int lpb = rows_lpb();
         a.lpb = lpb;
         a.lastsel = AxisSel{p.kdil[last] - 1, p.stride[last], p.out[last]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = size_t(2) * lpb * L * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es;
         const int64_t nb = (a.nlines + lpb - 1) / lpb;
         const double bytes = double(es) * double(a.nlines) * H + esz * double(p.y_numel);
         ctx.kbegin("fft_rows_inv", bytes, 5.0 * double(a.nlines) * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_inv_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((rows_inv_kernel<R, E>), dim3(unsigned(nb)), dim3(kThreads), smem, ctx.stream(), a);
+        hipLaunchKernelGGL((rows_inv_kernel<R, E>), dim3(unsigned(nb)), dim3(threads_for(int64_t(lpb) * L)), smem,
+                           ctx.stream(), a);
         ctx.kend();
         ndc_status s = check_launch(ctx, "rows inverse");
         if (s != NDC_OK) return s;

@@ -26,6 +26,12 @@ public:
     // Device table of W_n^i = exp(-2*pi*i*I/n), i < n, generated in FP64 on
     // the host and stored as float2 (dbl = false) or double2 (dbl = true).
     virtual const void* twiddles(int64_t n, bool dbl) = 0;
+    // Device copy of a constant table, uploaded once per key and kept for the
+    // handle's lifetime.
+    virtual const void* constant_table(const std::string& key, const void* data, size_t bytes) = 0;
+    // Per-handle host memo: a byte string kept under `key` for the handle's
+    // lifetime (empty the first time), for host-side plans built once.
+    virtual std::string& memo(const std::string& key) = 0;
     // Device copy of the plan's border maps, concatenated per axis.
     // offsets[i] is the element offset of axis i's map; the entry at
     // offsets[kMaxRank] is a single 0 used for virtual unit axes.
@@ -61,6 +67,10 @@ inline bool lds_fits(LaunchContext& ctx, const void* kernel, size_t dynamic_byte
 
 // Direct convolution (conv/mod.rs:128-200) in the element type.
 ndc_status launch_direct(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
+
+// Direct 2-D f32 convolution on the matrix cores in BF16 or split BF16x3
+// (p.algo), FP32 accumulation.
+ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
 
 // FFT convolution (conv_fft/mod.rs:185-292).
 ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);

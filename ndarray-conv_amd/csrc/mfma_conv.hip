@@ -1,0 +1,391 @@
+// Direct 2-D convolution of f32 data on the gfx950 matrix cores, with BF16
+// products (one, or three for the split hi/lo "BF16x3" form) and FP32
+// accumulation.  Same arithmetic contract as conv/mod.rs:177-197 (sum over the
+// taps of the reversed kernel), different rounding.
+//
+// A single-channel convolution has no channel dimension to make a GEMM of, so
+// the matrix product is taken over kernel ROWS:
+//   for one input row rho, P[m][r] = sum_c P(rho, (x0+m)*sx + c*dx) * kv(r, c)
+// is a 32x32 tile (m: 32 outputs along x, r: 32 kernel rows, c: kernel
+// columns as the K dimension), and P[.][r] belongs to output row
+// y = (rho - r*dy) / sy.  Streaming rho = (y0 + t)*sy, the partial sums
+// travel one lane (one kernel row) per step: the accumulator is shifted up by
+// one lane (DPP wave_shr:1) and fed back as the MFMA's C operand, so the lane
+// of the last kernel row holds a finished output row every step.  This needs
+// sy to divide dy; Q = dy/sy accumulator sets rotate when dy > sy.
+//
+// MFMA lane layout (v_mfma_f32_32x32x16_bf16): A lane l holds A[l%32][8*(l/32)+j],
+// B lane l holds B[8*(l/32)+j][l%32], D lane l holds D[8*(i/4)+4*(l/32)+i%4][l%32].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace ndc {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kXB = 32 * kWaves;     // outputs along x per workgroup
+constexpr int kMaxW = 256;           // staged row width limit (u32 entries)
+constexpr int kPer = 16 * kMaxW / kThreads;   // staged entries per thread per batch
+constexpr int kMaxChain = 1024;      // chain rows per pass (BY + 31 * Q)
+
+// input rows staged per batch: a multiple of Q keeps the accumulator-set
+// rotation aligned with the batches
+template <int Q> constexpr int rows_per_batch() { return Q == 3 ? 15 : 16; }
+
+struct MfmaArgs {
+    const float* x;
+    const float* k;
+    float* y;
+    const int32_t* map[2];   // border maps of the row and column axes
+    float cst[2][2];
+    int64_t xstr[2];
+    int64_t kstr[2];
+    int32_t OY, OX;          // output extents
+    int32_t KH, KW;          // kernel extents
+    int32_t sy, sx, dy, dx;
+    int32_t pdy, pdx;        // padded extents (map lengths)
+    int32_t rev;
+    int32_t BY;              // output rows per workgroup chain
+    int32_t ntx;             // x tiles
+    int32_t gx;              // gcd(sx, dx): staged columns are subsampled by it
+    int32_t W;               // staged row width (u32 entries)
+    int32_t KC;              // kernel columns per pass
+    int32_t npass_c;         // column passes
+    int32_t npass_r;         // row groups of 32
+};
+
+// Round to nearest even bf16 (the bit pattern in the low 16 bits).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// f = hi + lo (+ a residual of at most 2^-17 |f|), packed as hi | lo << 16.
+__device__ __forceinline__ uint32_t split_pack(float f) {
+    const uint32_t h = bf16_bits(f);
+    const float hf = __builtin_bit_cast(float, h << 16);
+    const uint32_t l = bf16_bits(f - hf);
+    return h | (l << 16);
+}
+
+// Value of the padded input from the border codes of its row and column
+// (codes resolved once per pass into LDS).
+__device__ __forceinline__ float pad_value(const MfmaArgs& a, int32_t cy, int32_t cx) {
+    // the higher axis' constant wins (later fills overwrite)
+    if (cx == kConstFront || cx == kConstBack) return a.cst[1][cx == kConstFront ? 0 : 1];
+    if (cy == kConstFront || cy == kConstBack) return a.cst[0][cy == kConstFront ? 0 : 1];
+    if (cx < 0 || cy < 0) return 0.0f;   // unwritten, or beyond the padded extent
+    return a.x[int64_t(cy) * a.xstr[0] + int64_t(cx) * a.xstr[1]];
+}
+
+template <int NKB, int Q, bool X3>
+__global__ void __launch_bounds__(kThreads) mfma_conv_kernel(MfmaArgs a) {
+    constexpr int kSR = rows_per_batch<Q>();
+    extern __shared__ uint32_t rows_lds[];   // [2][kSR][W] packed (hi, lo)
+    __shared__ int32_t s_row[kMaxChain];     // border code of chain row t
+    __shared__ int32_t s_col[kMaxW];         // border code of staged column j
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int half = lane >> 5;
+    const int m = lane & 31;
+    const int tx = blockIdx.x % a.ntx;
+    const int ty = blockIdx.x / a.ntx;
+    const int x0 = tx * kXB;
+    const int y0 = ty * a.BY;
+    const int xw = x0 + 32 * wave;           // this wave's first output column
+    const int sxg = a.sx / a.gx, dxg = a.dx / a.gx;
+    const int BYe = min(a.BY, a.OY - y0);
+
+    for (int pr = 0; pr < a.npass_r; ++pr) {
+        const int r0 = 32 * pr;
+        const int KHg = min(32, a.KH - r0);
+        for (int pc = 0; pc < a.npass_c; ++pc) {
+            const int c0 = pc * a.KC;
+            const int KCg = min(a.KC, a.KW - c0);
+            const bool first = pr == 0 && pc == 0;
+            // B operands: kernel rows r0 + (l % 32), columns c0 + 16 kb + 8 half + j
+            u32x4 bh[NKB], bl[NKB];
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+                uint32_t p[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int r = r0 + m;
+                    const int c = 16 * kb + 8 * half + j;
+                    float v = 0.0f;
+                    if (r < a.KH && c < KCg) {
+                        const int ki = a.rev ? a.KH - 1 - r : r;
+                        const int kj = a.rev ? a.KW - 1 - (c0 + c) : c0 + c;
+                        v = a.k[int64_t(ki) * a.kstr[0] + int64_t(kj) * a.kstr[1]];
+                    }
+                    p[j] = split_pack(v);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    bh[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+                    bl[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                }
+            }
+            // A-column validity: kernel columns beyond this pass contribute zero
+            uint32_t amask[NKB];
+#pragma unroll
+            for (int kb = 0; kb < NKB; ++kb) {
+                uint32_t mk = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (16 * kb + 8 * half + j < KCg) mk |= 1u << j;
+                amask[kb] = mk;
+            }
+            floatx16 acc[Q];
+#pragma unroll
+            for (int s = 0; s < Q; ++s)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[s][i] = 0.0f;
+
+            const int nsteps = BYe + (KHg - 1) * Q;
+            const int nbatch = (nsteps + kSR - 1) / kSR;
+            // staging: row t of the chain is input row (y0 + t) * sy + r0 * dy,
+            // columns x0*sx + c0*dx + j*gx, j < W
+            const int64_t pcol0 = int64_t(x0) * a.sx + int64_t(c0) * a.dx;
+            __syncthreads();   // the previous pass is done with s_row / s_col
+            for (int t = threadIdx.x; t < nsteps; t += kThreads)
+                s_row[t] = a.map[0][int64_t(y0 + t) * a.sy + int64_t(r0) * a.dy];
+            for (int j = threadIdx.x; j < a.W; j += kThreads) {
+                const int64_t px = pcol0 + int64_t(j) * a.gx;
+                s_col[j] = px < a.pdx ? a.map[1][px] : int32_t(kOutside);
+            }
+            __syncthreads();
+            const float rW = 1.0f / float(a.W);
+            auto fetch = [&](int b, uint32_t* regs) {
+#pragma unroll
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + i * kThreads;
+                    uint32_t v = 0;
+                    if (e < kSR * a.W) {
+                        int rr = int(float(e) * rW);
+                        int cc = e - rr * a.W;
+                        if (cc < 0) { --rr; cc += a.W; } else if (cc >= a.W) { ++rr; cc -= a.W; }
+                        const int t = b * kSR + rr;
+                        if (t < nsteps) v = split_pack(pad_value(a, s_row[t], s_col[cc]));
+                    }
+                    regs[i] = v;
+                }
+            };
+            uint32_t regs[kPer];   // host keeps W <= kMaxW, so kSR * W <= kPer * kThreads
+            fetch(0, regs);
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const int e = threadIdx.x + i * kThreads;
+                if (e < kSR * a.W) rows_lds[e] = regs[i];
+            }
+            __syncthreads();
+            int t = 0;
+            for (int b = 0; b < nbatch; ++b) {
+                if (b + 1 < nbatch) fetch(b + 1, regs);
+                const uint32_t* buf = rows_lds + (b & 1) * (kSR * a.W);
+                const int tend = min(nsteps, (b + 1) * kSR);
+                for (; t < tend;) {
+#pragma unroll
+                    for (int s = 0; s < Q; ++s, ++t) {
+                        if (t >= tend) break;
+                        const uint32_t* row = buf + (t - b * kSR) * a.W;
+                        // Hankel A operands from the staged row
+                        u32x4 ah[NKB], al[NKB];
+#pragma unroll
+                        for (int kb = 0; kb < NKB; ++kb) {
+                            const int base = (32 * wave + m) * sxg + (16 * kb + 8 * half) * dxg;
+                            uint32_t p[8];
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) p[j] = (amask[kb] >> j & 1u) ? row[base + j * dxg] : 0u;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+                                al[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                            }
+                        }
+                        // shift the accumulator one kernel row up; row 0 starts fresh
+                        floatx16 c;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            const int v = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, acc[s][i]), 0x138,
+                                                                      0xf, 0xf, true);
+                            c[i] = m == 0 ? 0.0f : __builtin_bit_cast(float, v);
+                        }
+#pragma unroll
+                        for (int kb = 0; kb < NKB; ++kb) {
+                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[kb]),
+                                                                       __builtin_bit_cast(bf16x8, bh[kb]), c, 0, 0, 0);
+                            if (X3) {
+                                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[kb]),
+                                                                           __builtin_bit_cast(bf16x8, bl[kb]), c, 0, 0, 0);
+                                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[kb]),
+                                                                           __builtin_bit_cast(bf16x8, bh[kb]), c, 0, 0, 0);
+                            }
+                        }
+                        acc[s] = c;
+                        // the lane of the last kernel row holds output row y
+                        const int yo = y0 + t - (KHg - 1) * Q;
+                        if (m == KHg - 1 && yo >= y0 && yo < y0 + BYe) {
+                            float* yr = a.y + int64_t(yo) * a.OX;
+#pragma unroll
+                            for (int i = 0; i < 16; ++i) {
+                                const int xo = xw + 8 * (i / 4) + 4 * half + (i % 4);
+                                if (xo < a.OX) yr[xo] = first ? c[i] : yr[xo] + c[i];
+                            }
+                        }
+                    }
+                }
+                if (b + 1 < nbatch) {
+                    uint32_t* nbuf = rows_lds + ((b + 1) & 1) * (kSR * a.W);
+#pragma unroll
+                    for (int i = 0; i < kPer; ++i) {
+                        const int e = threadIdx.x + i * kThreads;
+                        if (e < kSR * a.W) nbuf[e] = regs[i];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+int64_t gcd64(int64_t a, int64_t b) {
+    while (b) {
+        int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+template <int NKB, int Q, bool X3>
+const void* kernel_ptr() {
+    return reinterpret_cast<const void*>(&mfma_conv_kernel<NKB, Q, X3>);
+}
+
+template <int NKB, int Q, bool X3>
+void launch_one(const MfmaArgs& a, unsigned blocks, size_t smem, hipStream_t s) {
+    hipLaunchKernelGGL((mfma_conv_kernel<NKB, Q, X3>), dim3(blocks), dim3(kThreads), smem, s, a);
+}
+
+template <bool X3>
+const void* pick(int nkb, int q, void (**fn)(const MfmaArgs&, unsigned, size_t, hipStream_t)) {
+#define NDC_MFMA_CASE(NK, QQ)                          \
+    if (nkb == NK && q == QQ) {                        \
+        *fn = &launch_one<NK, QQ, X3>;                 \
+        return kernel_ptr<NK, QQ, X3>();               \
+    }
+    NDC_MFMA_CASE(1, 1) NDC_MFMA_CASE(2, 1) NDC_MFMA_CASE(3, 1) NDC_MFMA_CASE(4, 1)
+    NDC_MFMA_CASE(1, 2) NDC_MFMA_CASE(2, 2) NDC_MFMA_CASE(3, 2) NDC_MFMA_CASE(4, 2)
+    NDC_MFMA_CASE(1, 3) NDC_MFMA_CASE(2, 3) NDC_MFMA_CASE(3, 3) NDC_MFMA_CASE(4, 3)
+    NDC_MFMA_CASE(1, 4) NDC_MFMA_CASE(2, 4) NDC_MFMA_CASE(3, 4) NDC_MFMA_CASE(4, 4)
+#undef NDC_MFMA_CASE
+    return nullptr;
+}
+
+}  // namespace
+
+ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx) {
+    if (p.dtype != NDC_F32 || p.N != 2) {
+        ctx.err = "the BF16 matrix-core forms take rank-2 f32 convolutions";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    const int64_t sy = p.stride[0], sx = p.stride[1], dy = p.dil[0], dx = p.dil[1];
+    if (dy < 1 || dx < 1 || dy % sy != 0 || dy / sy > 4) {
+        ctx.err = "the BF16 matrix-core forms need dilation >= 1 and the row stride dividing the row dilation "
+                  "(at most 4x)";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    if (p.out[0] > (int64_t(1) << 30) || p.out[1] > (int64_t(1) << 30)) return NDC_ERR_UNSUPPORTED;
+    int64_t offs[kMaxRank + 1];
+    const int32_t* maps = ctx.border_maps(p, offs);
+    if (!maps) return NDC_ERR_OUT_OF_MEMORY;
+    MfmaArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.x = static_cast<const float*>(x);
+    a.k = static_cast<const float*>(k);
+    a.y = static_cast<float*>(y);
+    for (int i = 0; i < 2; ++i) {
+        a.map[i] = maps + offs[i];
+        a.cst[i][0] = float(p.side[i][0].value.re);
+        a.cst[i][1] = float(p.side[i][1].value.re);
+        a.xstr[i] = p.x_str[i];
+        a.kstr[i] = p.k_str[i];
+    }
+    a.OY = int32_t(p.out[0]);
+    a.OX = int32_t(p.out[1]);
+    a.KH = int32_t(p.k_shape[0]);
+    a.KW = int32_t(p.k_shape[1]);
+    a.sy = int32_t(sy);
+    a.sx = int32_t(sx);
+    a.dy = int32_t(dy);
+    a.dx = int32_t(dx);
+    a.pdy = int32_t(p.pds[0]);
+    a.pdx = int32_t(p.pds[1]);
+    a.rev = p.reverse ? 1 : 0;
+    const int Q = int(dy / sy);
+    a.gx = int32_t(gcd64(sx, dx));
+    // kernel columns per pass: up to 64 (four K-blocks of 16), fewer when the
+    // staged row (128 outputs plus the dilated column span) would not fit
+    int nkb = 0;
+    for (int cand = 4; cand >= 1; --cand) {
+        const int64_t w = ((kXB - 1) * sx + (16 * cand - 1) * dx) / a.gx + 1;
+        if (w <= kMaxW) {
+            nkb = cand;
+            a.W = int32_t(w);
+            break;
+        }
+    }
+    if (!nkb) {
+        ctx.err = "stride/dilation too large for the staged rows of the matrix-core form";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    nkb = std::min(nkb, int((a.KW + 15) / 16));
+    a.W = int32_t(((kXB - 1) * sx + (16 * nkb - 1) * dx) / a.gx + 1);
+    a.KC = std::min<int32_t>(a.KW, 16 * nkb);
+    a.npass_c = (a.KW + a.KC - 1) / a.KC;
+    a.npass_r = (a.KH + 31) / 32;
+    const int SR = Q == 3 ? 15 : 16;
+    // chain length: long chains amortise the (KH-1)*Q fill steps, short ones
+    // give more workgroups; aim for two workgroups per CU
+    a.ntx = int32_t((a.OX + kXB - 1) / kXB);
+    const int64_t target = 2 * int64_t(ctx.cu_count());
+    int64_t by = (int64_t(a.OY) * a.ntx + target - 1) / target;
+    const int64_t min_by = std::max<int64_t>(16, 2 * (std::min(a.KH, 32) - 1) * Q);
+    by = std::max(by, min_by);
+    by = std::min<int64_t>(by, a.OY);
+    by = std::min<int64_t>(by, kMaxChain - 31 * Q);   // the chain's row codes live in LDS
+    a.BY = int32_t(by);
+    const int64_t nty = (a.OY + a.BY - 1) / a.BY;
+    const int64_t blocks = nty * a.ntx;
+    const size_t smem = size_t(2) * SR * a.W * sizeof(uint32_t);
+    void (*fn)(const MfmaArgs&, unsigned, size_t, hipStream_t) = nullptr;
+    const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
+    const void* kp = x3 ? pick<true>(nkb, Q, &fn) : pick<false>(nkb, Q, &fn);
+    if (!kp || !fn) return NDC_ERR_UNSUPPORTED;
+    if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
+    // algorithmic work: 2 flops per output and tap (the FP32-equivalent count)
+    ctx.kbegin(x3 ? "mfma_conv_bf16x3" : "mfma_conv_bf16",
+               4.0 * double(p.x_numel + p.k_numel + p.y_numel), 2.0 * double(p.y_numel) * double(p.k_numel));
+    fn(a, unsigned(blocks), smem, ctx.stream());
+    ctx.kend();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx.err = std::string("matrix-core convolution launch: ") + hipGetErrorString(e);
+        return NDC_ERR_DEVICE;
+    }
+    return NDC_OK;
+}
+
+}  // namespace ndc

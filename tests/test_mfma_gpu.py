@@ -1,0 +1,99 @@
+"""The matrix-core precision forms of the 2-D f32 direct convolution.
+
+BF16x3 (split hi/lo, three products per tap, FP32 accumulation) must stay
+within the 1e-5 normwise relative error of the FP64 oracle that the plain
+FP32 path meets; single BF16 products are reported with a looser bound
+(about 2^-8 per product).  Shapes that the systolic form does not take
+(row stride not dividing the row dilation) must be refused, not computed."""
+import numpy as np
+import pytest
+
+from modes import conv_mode, padding_mode, random_border
+from oracle import ndconv_oracle as O
+
+import ndconv as nc
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"bf16x3": 1e-5, "bf16": 1e-2}
+
+
+def run(x, k, mode, pm, dil, rev, algo):
+    kw = nc.with_dilation(k, dil)
+    kw = kw if rev else kw.no_reverse()
+    ref = O.conv(x, k, mode, pm, dil, rev, acc_dtype=np.float64)   # raises first on invalid input
+    got = nc.conv(x, kw, conv_mode(mode), padding_mode(pm), algo=algo)
+    assert got.shape == ref.shape
+    return O.relerr(got, ref)[0]
+
+
+@pytest.mark.parametrize("algo", ["bf16x3", "bf16"])
+def test_mfma_random_cases(gpu, algo):
+    rng = np.random.default_rng(77)
+    worst = 0.0
+    for i in range(40):
+        ys, xs = int(rng.integers(1, 70)), int(rng.integers(1, 300))
+        kh, kw = int(rng.integers(1, 12)), int(rng.integers(1, 40))
+        sy = int(rng.integers(1, 3))
+        dy = sy * int(rng.integers(1, 4))          # sy divides dy
+        sx, dx = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+        mode = ("explicit", [[int(rng.integers(0, kh * dy)), int(rng.integers(0, kh * dy))],
+                             [int(rng.integers(0, kw * dx)), int(rng.integers(0, kw * dx))]], [sy, sx])
+        pm = ("explicit", [[random_border(rng, 0.5), random_border(rng, -0.25)] for _ in range(2)])
+        x = rng.standard_normal((ys, xs)).astype(np.float32)
+        k = rng.standard_normal((kh, kw)).astype(np.float32)
+        rev = bool(rng.integers(0, 2))
+        try:
+            r = run(x, k, mode, pm, [dy, dx], rev, algo)
+        except O.OracleError as e:
+            with pytest.raises(getattr(nc, e.kind)):
+                nc.conv(x, nc.with_dilation(k, [dy, dx]), conv_mode(mode), padding_mode(pm), algo=algo)
+            continue
+        assert r <= TOL[algo], (i, r)
+        worst = max(worst, r)
+    print(f"{algo}: worst normwise rel err {worst:.3e}")
+
+
+@pytest.mark.parametrize("kshape,dil", [((40, 7), (1, 1)), ((33, 70), (1, 2)), ((5, 130), (2, 1)),
+                                        ((9, 9), (4, 3))])
+def test_mfma_row_groups_and_column_passes(gpu, kshape, dil):
+    # more than 32 kernel rows (row groups), more than 64 columns (column
+    # passes) and dilation/stride ratios 2..4 (rotating accumulator sets)
+    rng = np.random.default_rng(sum(kshape))
+    x = rng.standard_normal((150, 400)).astype(np.float32)
+    k = rng.standard_normal(kshape).astype(np.float32)
+    for mode in ["same", "valid", "full"]:
+        try:
+            r = run(x, k, mode, "reflect", list(dil), True, "bf16x3")
+        except O.OracleError:
+            continue
+        assert r <= 1e-5, (mode, r)
+
+
+def test_mfma_workload4(gpu):
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((4096, 4096)).astype(np.float32)
+    k = rng.standard_normal((31, 31)).astype(np.float32)
+    mode = nc.ConvMode.Custom([0, 0], [2, 2])
+    ref = O.conv_fft(x.astype(np.float64), k.astype(np.float64), ("custom", [0, 0], [2, 2]), "replicate", 2)
+    errs = {}
+    for algo in ["direct", "bf16x3", "bf16"]:
+        got = nc.conv(x, nc.with_dilation(k, 2), mode, nc.PaddingMode.Replicate, algo=algo)
+        errs[algo] = O.relerr(got, ref)[0]
+    print("workload4 normwise rel err:", errs)
+    assert errs["direct"] <= 1e-5
+    assert errs["bf16x3"] <= 1e-5
+    assert errs["bf16"] <= 1e-2
+
+
+def test_mfma_refuses_unsupported(gpu):
+    x = np.ones((20, 20), np.float32)
+    k = np.ones((3, 3), np.float32)
+    with pytest.raises(nc.Unsupported):   # row stride 2 does not divide row dilation 1
+        nc.conv(x, k, nc.ConvMode.Custom([1, 1], [2, 1]), nc.PaddingMode.Zeros, algo="bf16x3")
+    with pytest.raises(nc.Unsupported):   # rank 1
+        nc.conv(np.ones(20, np.float32), np.ones(3, np.float32), nc.ConvMode.Same, nc.PaddingMode.Zeros,
+                algo="bf16x3")
+    with pytest.raises(nc.Unsupported):   # f64
+        nc.conv(x.astype(np.float64), k.astype(np.float64), nc.ConvMode.Same, nc.PaddingMode.Zeros,
+                algo="bf16x3")
