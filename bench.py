@@ -62,6 +62,23 @@ def workloads():
     }
 
 
+def uneven_lengths(seed=5, count=4096):
+    """Signal lengths log-uniform in [256, 262144] and kernel lengths
+    log-uniform in [3, 1023] (floor of exp of a uniform draw, clipped)."""
+    rng = np.random.default_rng(seed)
+    n = np.clip(np.floor(np.exp(rng.uniform(np.log(256), np.log(262145), count))), 256, 262144).astype(np.int64)
+    k = np.clip(np.floor(np.exp(rng.uniform(np.log(3), np.log(1024), count))), 3, 1023).astype(np.int64)
+    return n, k, rng
+
+
+def uneven_batch(seed=5, count=4096):
+    """The uneven batch: lengths as above, values Uniform[-1, 1), packed."""
+    n, k, rng = uneven_lengths(seed, count)
+    x = rng.uniform(-1.0, 1.0, int(n.sum())).astype(np.float32)
+    kv = rng.uniform(-1.0, 1.0, int(k.sum())).astype(np.float32)
+    return n, k, x, kv
+
+
 def make_inputs(w):
     rng = np.random.default_rng(w["seed"])
     dt = np.dtype(w["dtype"])
@@ -98,6 +115,135 @@ def cpu_baseline(w, x, k, budget_s=12.0):
                       f"oracle float32 restatement of conv_fft with scipy pocketfft, workers={cores}"}
 
 
+def batch_cpu_baseline(n, k, x, kv, budget_s=12.0):
+    """The oracle's f32 conv_fft restatement per pair (scipy pocketfft), the
+    pairs spread over this host's cores, on a bounded sample of the batch."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import ndconv_oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    xo = np.concatenate([[0], np.cumsum(n)])
+    ko = np.concatenate([[0], np.cumsum(k)])
+    order = np.random.default_rng(0).permutation(len(n))
+
+    def one(i):
+        return O.conv_fft(x[xo[i]:xo[i + 1]], kv[ko[i]:ko[i + 1]], "same", "circular", precision="f32").size
+
+    done, outs = 0, 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        while time.perf_counter() - t0 < budget_s and done < len(order):
+            chunk = order[done:done + 4 * cores]
+            outs += sum(ex.map(one, chunk))
+            done += len(chunk)
+    el = time.perf_counter() - t0
+    return {"value": outs / el, "unit": "outputs/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of {len(n)} pairs drawn at random ({outs} outputs, {el:.1f} s), oracle float32 "
+                      f"restatement of conv_fft per pair with scipy pocketfft, {cores} pairs in flight"}
+
+
+def run_batch(args):
+    """The uneven batch: 4096 rank-1 f32 pairs, one persistent launch per
+    step, replayed through a HIP graph."""
+    import torch
+    import ndconv as nc
+    from ndconv import _lib as C
+
+    n, k, x, kv = uneven_batch(5, 4096)
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(x).to(dev)
+    kt = torch.from_numpy(kv).to(dev)
+    proc = nc.get_fft_processor()
+    h = proc.handle
+    lib = C.lib()
+    t0 = time.perf_counter()
+    bp = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo=args.batch_algo, processor=proc)
+    plan_s = time.perf_counter() - t0
+    yt = torch.empty(bp.y_total, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(graph, steps):
+        for _ in range(args.warmup):
+            bp.run(xt, kt, yt, graph=graph)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            bp.run(xt, kt, yt, graph=graph)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    t0 = time.perf_counter()
+    ms_graph = timed(True, args.steps)
+    wall = time.perf_counter() - t0
+    ms_eager = timed(False, args.steps)
+    outputs = int(bp.y_total)
+
+    # sampled accuracy against the FP64 oracle
+    from oracle import ndconv_oracle as O
+    y = yt.cpu().numpy()
+    xo, ko, yo = (np.concatenate([[0], np.cumsum(a)]) for a in (n, k, bp.y_lens))
+    worst = 0.0
+    for i in np.random.default_rng(1).choice(len(n), 16, replace=False):
+        ref = O.conv_fft(x[xo[i]:xo[i + 1]].astype(np.float64), kv[ko[i]:ko[i + 1]].astype(np.float64), "same",
+                         "circular")
+        worst = max(worst, O.relerr(y[yo[i]:yo[i + 1]], ref)[0])
+
+    # dominant kernel: HIP events around each launch on the library's stream
+    nc._check(lib.ndc_profile_enable(h, 1), h)
+    for _ in range(args.steps):
+        bp.run(xt, kt, yt)
+    buf = ctypes.create_string_buffer(1 << 16)
+    nc._check(lib.ndc_profile_report(h, buf, len(buf)), h)
+    nc._check(lib.ndc_profile_enable(h, 0), h)
+    prof = json.loads(buf.value.decode())
+    dom_name, dom = max(prof.items(), key=lambda kv_: kv_[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bpl = dom["bytes"] / dom["launches"]
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name, "avg_ms": avg_ms,
+            "bytes_per_launch": bpl}
+    kernels = {nm: {"avg_ms": v["total_ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
+                    "GB/s": (v["bytes"] / v["launches"]) / (v["total_ms"] / v["launches"] * 1e-3) / 1e9}
+               for nm, v in prof.items()}
+    report = bp.report()
+
+    # the unbatched path: one single-problem conv_fft per pair
+    plain = None
+    if args.plain_steps > 0:
+        bq = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo="plain", processor=proc)
+        bq.run(xt, kt, yt)
+        torch.cuda.synchronize()
+        p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for _ in range(args.plain_steps):
+            bq.run(xt, kt, yt)
+        p1.record(stream)
+        torch.cuda.synchronize()
+        plain = p0.elapsed_time(p1) / args.plain_steps
+        bq.close()
+
+    cpu = None if args.no_cpu else batch_cpu_baseline(n, k, x, kv)
+    value = outputs / (ms_graph * 1e-3)
+    line = {"metric": METRIC, "value": value, "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_graph, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "uneven batch: 4096 rank-1 f32 pairs, n log-uniform [256, 262144], "
+                                   "k log-uniform [3, 1023], Uniform[-1,1), Same, Circular",
+                       "outputs_per_step": outputs, "seed": 5, "algo": args.batch_algo},
+            "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall,
+            "eager_value": outputs / (ms_eager * 1e-3), "ms_per_step_eager": ms_eager,
+            "max_rel_err_sampled": worst, "plan_s": plan_s,
+            "schedule": {kk: report[kk] for kk in ("items", "lines", "kernel_spectra", "grid", "blocks_per_cu",
+                                                   "costs")}}
+    if plain is not None:
+        line["plain_ms_per_step"] = plain
+        line["plain_value"] = outputs / (plain * 1e-3)
+        line["speedup_vs_plain"] = plain / ms_graph
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,10 +252,15 @@ def main():
     ap.add_argument("--workload", default="fft2d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step")
+    ap.add_argument("--batch-algo", default="auto", help="uneven batch: auto, fft, direct")
+    ap.add_argument("--plain-steps", type=int, default=2, help="uneven batch: steps of the unbatched path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+    if args.workload == "batch1d":
+        run_batch(args)
+        return
 
     import torch
     import ndconv as nc

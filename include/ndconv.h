@@ -154,7 +154,8 @@ typedef struct ndc_handle_s* ndc_handle;
 
 /* Library identification.  ndc_struct_size lets an FFI binding check its
  * struct layouts: 0 ndc_conv_desc, 1 ndc_plan_info, 2 ndc_conv_mode,
- * 3 ndc_padding_mode, 4 ndc_border, 5 ndc_value; -1 for anything else.
+ * 3 ndc_padding_mode, 4 ndc_border, 5 ndc_value, 6 ndc_batch_desc; -1 for
+ * anything else.
  * (No reference counterpart: FFI hygiene for the shims.) */
 int32_t ndc_abi_version(void);
 int64_t ndc_struct_size(int32_t which);
@@ -203,6 +204,68 @@ ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, 
  * those launches; then it clears the records. */
 ndc_status ndc_profile_enable(ndc_handle h, int32_t on);
 ndc_status ndc_profile_report(ndc_handle h, char* buf, int64_t len);
+
+/* ---------------------------------------------------------------------------
+ * Batches of independent rank-1 f32 signal/kernel pairs of uneven lengths.
+ *
+ * The reference has no batch call; a caller of it loops over the pairs and
+ * calls ConvFFTExt::conv_fft (conv_fft/mod.rs:113-171) once per pair.  Each
+ * pair here has exactly that call's semantics and errors (ConvMode, PaddingMode,
+ * dilation and reverse shared by the batch), but the whole batch runs as one
+ * persistent launch draining a device-side work queue.  Signals, kernels and
+ * outputs are packed back to back in pair order (x of pair i starts at
+ * sum_{j<i} x_len[j], likewise k and y); query the output lengths with
+ * ndc_batch_out_lens().
+ * ------------------------------------------------------------------------- */
+
+typedef enum ndc_batch_algo {
+    NDC_BATCH_AUTO = 0,     /* per pair, the cheaper of direct and overlap-save FFT by the
+                             * device-measured cost model (the autotuner) */
+    NDC_BATCH_FFT = 1,      /* overlap-save FFT for every pair whose dilated kernel fits
+                             * the largest block (8192 samples), direct otherwise */
+    NDC_BATCH_DIRECT = 2,   /* direct FP32 for every pair */
+    NDC_BATCH_PLAIN = 3     /* one single-problem conv_fft launch sequence per pair (the
+                             * unbatched path, for comparison); run_graph runs it eagerly */
+} ndc_batch_algo;
+
+typedef struct ndc_batch_desc {
+    int32_t dtype;                  /* NDC_F32 */
+    int32_t algo;                   /* ndc_batch_algo */
+    int64_t count;                  /* number of pairs */
+    const int64_t* x_len;           /* [count] signal lengths */
+    const int64_t* k_len;           /* [count] kernel lengths */
+    int64_t dilation;               /* KernelWithDilation (dilation/mod.rs:64-80) */
+    int32_t reverse;                /* 1 convolution, 0 no_reverse (dilation/mod.rs:117-189) */
+    uint32_t flags;                 /* NDC_FLAG_HOST_MEMORY */
+    ndc_conv_mode conv_mode;        /* rank-1 ConvMode, shared */
+    ndc_padding_mode padding_mode;  /* rank-1 PaddingMode, shared */
+} ndc_batch_desc;
+
+typedef struct ndc_batch_s* ndc_batch;
+
+/* Validates every pair (first failing pair: its status, and its index in
+ * ndc_last_error), plans the work queue and uploads it.  AUTO measures the
+ * cost model on the device the first time a handle needs it.  The batch
+ * keeps using h (stream, profile) and must be destroyed before it. */
+ndc_status ndc_batch_create(ndc_handle h, const ndc_batch_desc* desc, ndc_batch* out);
+ndc_status ndc_batch_destroy(ndc_batch b);
+/* y_len[count] (may be NULL) and the packed totals (each may be NULL). */
+ndc_status ndc_batch_out_lens(ndc_batch b, int64_t* y_len, int64_t* x_total, int64_t* k_total, int64_t* y_total);
+/* Enqueues the batch on the handle's stream (device pointers; with
+ * NDC_FLAG_HOST_MEMORY host pointers and synchronous).  No allocation and no
+ * host synchronisation in device mode, so it may be captured in a graph. */
+ndc_status ndc_batch_run(ndc_batch b, const void* x, const void* k, void* y);
+/* Same result through a HIP graph of the batch's launches, captured on the
+ * first call and replayed while x, k, y stay the same (re-captured when they
+ * change). */
+ndc_status ndc_batch_run_graph(ndc_batch b, const void* x, const void* k, void* y);
+/* JSON: the measured cost constants, the choice per kernel-length and
+ * signal-length bucket, item counts and algorithmic bytes per launch. */
+ndc_status ndc_batch_report(ndc_batch b, char* buf, int64_t len);
+/* Border map of pair `pair` over its padded length (codes as ndc_border_map
+ * with NDC_OP_CONV) from the closed form the batch kernels evaluate per
+ * sample. */
+ndc_status ndc_batch_border_map(const ndc_batch_desc* desc, int64_t pair, int32_t* codes, int64_t len);
 
 #ifdef __cplusplus
 }

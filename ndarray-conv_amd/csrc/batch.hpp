@@ -1,0 +1,105 @@
+// Batches of independent rank-1 signal/kernel pairs of uneven lengths, run
+// as one persistent launch that drains a device-side work queue.
+//
+// Every pair has the semantics of one rank-1 conv_fft call
+// (conv_fft/mod.rs:185-292): ConvMode and PaddingMode are shared by the
+// batch, lengths differ per pair.  Each pair is computed by one of
+//   - direct items: a tile of outputs, FP32 multiply-adds over the taps;
+//   - overlap-save FFT items: blocks of outputs, each from one real FFT of
+//     length L = 2 * LC (LC = 256 .. 4096 complex points) of a window of the
+//     padded signal, multiplied by the pair's kernel spectrum (computed by a
+//     pre-pass at the same L) and transformed back.  Overlap-save computes the
+//     same linear correlation as the reference's single FFT of the whole padded
+//     signal, only blocked: window position u >= kdil - 1 of a block holds
+//     full-correlation index block_start + u.
+// The choice per pair comes from a cost model whose constants are measured
+// on the device (the autotuner).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+#include "ndconv.h"
+
+namespace ndc {
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchClasses = 5;             // LC = 256 << c, c = 0..4
+constexpr int kBatchPoints = 4096;           // complex points per FFT item (G lines of LC)
+constexpr int kBatchDirectTile = 2048;       // outputs per direct item (8 per thread)
+constexpr int kBatchWindow = 6144;           // LDS window capacity of a direct item (floats)
+constexpr int kBatchTapChunk = 1024;         // taps staged per round of a direct item
+
+constexpr int batch_lc(int cls) { return 256 << cls; }
+constexpr int batch_lines(int cls) { return kBatchPoints / batch_lc(cls); }
+// Quarter-circle twiddle tables of all classes, concatenated: class c has
+// LC/2 = 128 << c entries.
+constexpr int batch_tw_offset(int cls) { return 128 * ((1 << cls) - 1); }
+
+// Per-pair metadata read by the kernels (64 bytes).
+struct BatchPair {
+    int64_t x_off, k_off, y_off;   // element offsets into the packed x, k, y
+    int64_t ks_off;                // complex offset of the kernel spectrum (FFT pairs)
+    int32_t n, k;                  // signal and kernel length
+    int32_t p0, pds;               // front pad, padded length n + p0 + p1
+    int32_t out;                   // output length
+    int32_t per;                   // outputs per FFT block
+    int32_t cls;                   // FFT class, or -1 for direct
+    int32_t pad_;
+};
+
+// One unit of the work queue (16 bytes).
+//   kind 0: direct tile  a = pair, b = first output, c = output count
+//   kind 1 + cls: FFT    a = first line, b = line count (<= batch_lines(cls))
+struct BatchItem {
+    int32_t kind, a, b, c;
+};
+
+// One overlap-save block: pair and first output index.
+struct BatchLine {
+    int32_t pair, o0;
+};
+
+// Kernel-spectrum pre-pass work: up to batch_lines(cls) pairs of one class.
+struct BatchKItem {
+    int32_t cls, first, count, pad_;
+};
+
+// Shared launch parameters of a batch.
+struct BatchArgs {
+    const float* x;
+    const float* k;
+    float* y;
+    const BatchPair* pairs;
+    const BatchItem* items;
+    const BatchLine* lines;
+    const int32_t* kpairs;         // pairs of the kernel-spectrum pre-pass, grouped by class
+    const BatchKItem* kitems;
+    float2* kspec;
+    int32_t* counter;              // work-queue head, reset by the pre-pass
+    const float2* tw;              // W_L^r, r < L/4, of class c at batch_tw_offset(c) (L = 2 LC)
+    int32_t nitems, nkitems;
+    int32_t s, d, rev;
+    int32_t front, back;           // side tags of border1d.hpp
+    float cfront, cback;           // constants of Const sides
+};
+
+// Device launchers (batch_conv.hip).
+size_t batch_lds_bytes();
+int batch_blocks_per_cu();
+// Pre-pass then persistent main launch on `stream`.  With `profile` each
+// launch is bracketed for ctx's per-kernel profile (not while capturing);
+// the byte/flop counts are the algorithmic traffic of each launch.
+ndc_status launch_batch(const BatchArgs& a, int grid, LaunchContext& ctx, hipStream_t stream, bool profile,
+                        double kspec_bytes, double main_bytes, double main_flops);
+
+// Services of the handle behind an ndc_handle (capi.cpp).
+LaunchContext* handle_context(ndc_handle h);
+int handle_device(ndc_handle h);
+void handle_set_last(ndc_handle h, const std::string& msg);
+
+}  // namespace ndc

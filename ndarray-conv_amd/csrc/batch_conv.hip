@@ -1,0 +1,467 @@
+// Kernels of the batched rank-1 path: a kernel-spectrum pre-pass and one
+// persistent launch that drains the work queue (direct tiles and
+// overlap-save FFT blocks of all pairs).
+//
+// FFT items hold kBatchPoints complex points in LDS: G = kBatchPoints / LC
+// lines of LC points, each line the half-length complex view
+// z[n] = w[2n] + i w[2n+1] of a real window of L = 2 LC samples.  A line is
+// transformed by radix-16 Stockham passes with the 16 points of a butterfly
+// in registers (the last pass is radix 2/4/8/16 to reach LC), LDS only being
+// used to exchange between passes.  The padded layout i + i/16 keeps the
+// stride-16 accesses of the first pass free of bank conflicts.  The window is
+// read from global memory with the border remap fused into the load, and only
+// the cropped, strided outputs are written back.  Twiddles W_L^r come from a
+// quarter-circle table generated in FP64 on the host and staged in LDS.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "border1d.hpp"
+#include "fft_bfly.hpp"
+
+namespace ndc {
+namespace {
+
+using namespace fftb;
+using C = float2;
+
+constexpr int T = kBatchThreads;
+constexpr int kTwSlots = 2048;                                   // L/4 for the largest class
+constexpr int kDataSlots = kBatchPoints + kBatchPoints / 16;     // padded FFT data
+constexpr int kWinSlots = kBatchWindow + kBatchWindow / 8;       // padded direct window
+constexpr size_t kLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
+static_assert(size_t(kWinSlots + kBatchTapChunk) * sizeof(float) <= size_t(kDataSlots) * sizeof(C),
+              "direct-item staging must fit the FFT data region");
+
+// threadIdx.x behind an empty volatile asm: inside the persistent loop it is
+// re-read per item, so the compiler cannot hoist the address arithmetic of
+// every item type out of the loop and keep it live in registers.
+__device__ __forceinline__ int tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+__device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int padw(int i) { return i + (i >> 3); }
+
+// W_L^e from the quarter table q[r] = W_L^r, r < L/4 (L a power of two).
+__device__ __forceinline__ C twl(const C* q, int e, int L) {
+    e &= L - 1;
+    const int q4 = L >> 2;
+    const int quad = e / q4;
+    const C v = q[e & (q4 - 1)];
+    switch (quad) {
+        case 0: return v;
+        case 1: return cmake<C>(v.y, -v.x);     // * -i
+        case 2: return cmake<C>(-v.x, -v.y);    // * -1
+        default: return cmake<C>(-v.y, v.x);    // * i
+    }
+}
+
+// v[r] *= W_L^(r E) (conjugated for the inverse), r = 1 .. R-1; powers
+// beyond 1, 2, 4, 8 are products of at most three looked-up factors.
+template <int R>
+__device__ __forceinline__ void twiddle(C* v, int E, const C* q, int L, bool inv) {
+    C w[16];
+    w[1] = twl(q, E, L);
+    if constexpr (R > 2) {
+        w[2] = twl(q, 2 * E, L);
+        w[3] = cmul(w[2], w[1]);
+    }
+    if constexpr (R > 4) {
+        w[4] = twl(q, 4 * E, L);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R > 8) {
+        w[8] = twl(q, 8 * E, L);
+#pragma unroll
+        for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmulc(v[r], w[r], inv);
+}
+
+// One radix-R Stockham pass, in place, over the G lines of an item in LDS:
+// butterfly j of a line reads points j + r LC/R, applies W_(Ns R)^(r k)
+// (k = j mod Ns), and writes (j - k) R + k + r Ns.  Ends with a barrier.
+template <int LC, int R>
+__device__ __forceinline__ void pass(C* buf, const C* q, int Ns, bool inv) {
+    constexpr int NB = 16 / R;
+    constexpr int BPL = LC / R;
+    constexpr int LS = LC + LC / 16;
+    constexpr int L = 2 * LC;
+    C v[16];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int beta = tid() + b * T;
+        const int g = beta / BPL, j = beta % BPL;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b * R + r] = buf[g * LS + padi(j + r * BPL)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int beta = tid() + b * T;
+        const int g = beta / BPL, j = beta % BPL;
+        const int k = j & (Ns - 1);
+        if (Ns > 1) twiddle<R>(v + b * R, k * (L / (Ns * R)), q, L, inv);
+        bfly<R>(v + b * R, inv);
+        const int ob = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[g * LS + padi(ob + r * Ns)] = v[b * R + r];
+    }
+    __syncthreads();
+}
+
+template <int LC> struct LastRadix { static constexpr int value = LC / 256; };
+template <> struct LastRadix<256> { static constexpr int value = 16; };
+
+// Transform of the G lines in LDS (natural order in and out): radix 16,
+// radix 16, then the radix that completes LC.
+template <int LC>
+__device__ __forceinline__ void fft_lds(C* buf, const C* q, bool inv) {
+    pass<LC, 16>(buf, q, 1, inv);
+    pass<LC, 16>(buf, q, 16, inv);
+    if constexpr (LC > 256) pass<LC, LastRadix<LC>::value>(buf, q, 256, inv);
+}
+
+// Real sample u of line g in the padded layout (the complex view's point
+// u/2, real part for even u).
+template <int LC>
+__device__ __forceinline__ int real_slot(int g, int u) {
+    constexpr int LS = LC + LC / 16;
+    return 2 * (g * LS + padi(u >> 1)) + (u & 1);
+}
+
+// X[m] of a real line from its half-length spectrum: zk = Z[m], zo = Z[LC-m].
+__device__ __forceinline__ C r2c(C zk, C zo, C w) {
+    const C zc = cconj(zo);
+    const C e = cscale(cadd(zk, zc), 0.5f);
+    const C o = cmul_negi(cscale(csub(zk, zc), 0.5f));
+    return cadd(e, cmul(o, w));
+}
+
+// Half-length inverse input from A = P[m], B = conj P[LC-m].
+__device__ __forceinline__ C c2r(C A, C B, C w) {
+    return cadd(cadd(A, B), cmul_i(cmul(csub(A, B), cconj(w))));
+}
+
+// Per-line context of an item, staged in LDS.
+struct LineCtx {
+    int64_t x_off, y_off, ks_off, k_off;
+    int32_t n, k, p0, pds, out, per, o0, valid;
+};
+
+__device__ __forceinline__ float pad_value(const BatchArgs& a, const LineCtx& c, int64_t P) {
+    if (P >= c.pds) return 0.f;
+    const int64_t r = P - c.p0;
+    if (r >= 0 && r < c.n) return a.x[c.x_off + r];
+    const int64_t s = resolve1d(P, c.n, c.p0, a.front, a.back);
+    if (s >= 0) return a.x[c.x_off + s];
+    if (s == kConstFront) return a.cfront;
+    if (s == kConstBack) return a.cback;
+    return 0.f;
+}
+
+// Tap t of the correlation form out[o] = sum_t pad[o s + t d] kc[t]:
+// kc[t] = k[K-1-t] for a convolution (reverse), k[t] for no_reverse.
+__device__ __forceinline__ float corr_tap(const BatchArgs& a, int64_t k_off, int K, int t) {
+    return a.k[k_off + (a.rev ? K - 1 - t : t)];
+}
+
+__device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, int G, int first, int count, bool kernel_lines) {
+    const int g = tid();
+    if (g < G) {
+        LineCtx c{};
+        c.valid = g < count;
+        if (c.valid) {
+            int pi;
+            if (kernel_lines) {
+                pi = a.kpairs[first + g];
+                c.o0 = 0;
+            } else {
+                const BatchLine ln = a.lines[first + g];
+                pi = ln.pair;
+                c.o0 = ln.o0;
+            }
+            const BatchPair p = a.pairs[pi];
+            c.x_off = p.x_off;
+            c.y_off = p.y_off;
+            c.ks_off = p.ks_off;
+            c.k_off = p.k_off;
+            c.n = p.n;
+            c.k = p.k;
+            c.p0 = p.p0;
+            c.pds = p.pds;
+            c.out = p.out;
+            c.per = p.per;
+        }
+        lc[g] = c;
+    }
+}
+
+// Writes the real samples of the G lines: sample u of line g is val(c, u)
+// for a valid line, 0 otherwise.  kU loads in flight per thread.
+template <int LC, typename F>
+__device__ __forceinline__ void load_reals(float* fb, const LineCtx* lc, F&& val) {
+    constexpr int G = kBatchPoints / LC;
+    constexpr int L = 2 * LC;
+    constexpr int kU = 8;
+    for (int base = tid(); base < G * L; base += kU * T) {
+        float v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = base + u * T;   // G * L is a multiple of kU * T
+            const LineCtx& c = lc[idx / L];
+            v[u] = c.valid ? val(c, idx % L) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = base + u * T;
+            fb[real_slot<LC>(idx / L, idx % L)] = v[u];
+        }
+    }
+    __syncthreads();
+}
+
+template <int LC>
+__device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it, const C* q, C* buf, LineCtx* lc) {
+    constexpr int G = kBatchPoints / LC;
+    constexpr int LS = LC + LC / 16;
+    constexpr int L = 2 * LC;
+    constexpr int H = LC / 2 + 1;
+    load_line_ctx(a, lc, G, it.a, it.b, false);
+    __syncthreads();
+    const int s = a.s;
+    float* fb = reinterpret_cast<float*>(buf);
+    // window of padded positions [o0 s, o0 s + L), border remap fused in
+    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) { return pad_value(a, c, int64_t(c.o0) * s + u); });
+    fft_lds<LC>(buf, q, false);
+    // spectrum product, bins m and LC - m together
+    for (int idx = tid(); idx < G * H; idx += T) {
+        const int g = idx / H, m = idx - g * H;
+        const LineCtx& c = lc[g];
+        if (!c.valid) continue;
+        C* Z = buf + g * LS;
+        const int mp = LC - m;
+        const C zm = Z[padi(m)];
+        const C zp = Z[padi(mp == LC ? 0 : mp)];
+        const C wm = twl(q, m, L), wp = twl(q, mp, L);
+        const C* K = a.kspec + c.ks_off;
+        C Pm = cmul(r2c(zm, zp, wm), K[m]);
+        C Pp = cmul(r2c(zp, zm, wp), K[mp]);
+        if (m == 0) {   // imaginary parts of DC and Nyquist are ignored by c2r
+            Pm.y = 0.f;
+            Pp.y = 0.f;
+        }
+        Z[padi(m)] = c2r(Pm, cconj(Pp), wm);
+        if (mp != m && mp < LC) Z[padi(mp)] = c2r(Pp, cconj(Pm), wp);
+    }
+    __syncthreads();
+    fft_lds<LC>(buf, q, true);
+    // crop and stride: output o0 + j is window sample kdil - 1 + j s
+    for (int g = 0; g < G; ++g) {
+        const LineCtx& c = lc[g];
+        if (!c.valid) continue;
+        const int kdil = (c.k - 1) * a.d + 1;
+        const int cnt = c.per < c.out - c.o0 ? c.per : c.out - c.o0;
+        float* y = a.y + c.y_off + c.o0;
+        for (int j = tid(); j < cnt; j += T) y[j] = fb[real_slot<LC>(g, kdil - 1 + j * s)];
+    }
+}
+
+template <int LC>
+__device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem& it, const C* q, C* buf, LineCtx* lc) {
+    constexpr int G = kBatchPoints / LC;
+    constexpr int LS = LC + LC / 16;
+    constexpr int L = 2 * LC;
+    load_line_ctx(a, lc, G, it.first, it.count, true);
+    __syncthreads();
+    const int d = a.d;
+    float* fb = reinterpret_cast<float*>(buf);
+    // scattered kernel of the reference's FFT path (conv_fft/padding.rs:78-111):
+    // k[i] at i d (reverse) or (K-1-i) d (no_reverse)
+    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) -> float {
+        if (u % d) return 0.f;
+        const int i = u / d;
+        if (i >= c.k) return 0.f;
+        return a.k[c.k_off + (a.rev ? i : c.k - 1 - i)];
+    });
+    fft_lds<LC>(buf, q, false);
+    const float scale = 1.0f / float(L);   // the inverse normalisation, folded in
+    for (int idx = tid(); idx < G * (LC + 1); idx += T) {
+        const int g = idx / (LC + 1), m = idx - g * (LC + 1);
+        const LineCtx& c = lc[g];
+        if (!c.valid) continue;
+        const C* Z = buf + g * LS;
+        const C zm = Z[padi(m == LC ? 0 : m)];
+        const C zp = Z[padi(m == 0 ? 0 : LC - m)];
+        a.kspec[c.ks_off + m] = cscale(r2c(zm, zp, twl(q, m, L)), scale);
+    }
+}
+
+// Direct tile: outputs [o0, o0 + cnt) of one pair, taps staged in chunks.
+__device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem& it, float* win, float* taps) {
+    const BatchPair p = a.pairs[it.a];
+    const int o0 = it.b, cnt = it.c;
+    const int s = a.s, d = a.d;
+    const int K = p.k;
+    const bool fast = s == 1 && d == 1;
+    const int span = (cnt - 1) * s + 1;
+    if (span > kBatchWindow / 2) return;   // the planner sizes tiles below this
+    int chunk = (kBatchWindow - span) / d + 1;
+    if (chunk > kBatchTapChunk) chunk = kBatchTapChunk;
+    float acc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = 0.f;
+    LineCtx c{};
+    c.x_off = p.x_off;
+    c.n = p.n;
+    c.p0 = p.p0;
+    c.pds = p.pds;
+    for (int t0 = 0; t0 < K; t0 += chunk) {
+        const int nt = K - t0 < chunk ? K - t0 : chunk;
+        const int64_t wbase = int64_t(o0) * s + int64_t(t0) * d;
+        const int wlen = span + (nt - 1) * d;
+        for (int e = tid(); e < wlen; e += T) win[padw(e)] = pad_value(a, c, wbase + e);
+        for (int t = tid(); t < nt; t += T) taps[t] = corr_tap(a, p.k_off, K, t0 + t);
+        __syncthreads();
+        if (fast) {
+            // 8 consecutive outputs per thread, a sliding register window
+            const int ob = 8 * tid();
+            float r[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) r[m] = win[padw(ob + m)];
+            for (int t = 0; t < nt; ++t) {
+                const float kv = taps[t];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) acc[m] += r[m] * kv;
+#pragma unroll
+                for (int m = 0; m < 7; ++m) r[m] = r[m + 1];
+                r[7] = win[padw(ob + t + 8)];
+            }
+        } else {
+            for (int t = 0; t < nt; ++t) {
+                const float kv = taps[t];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int o = tid() + m * T;
+                    if (o < cnt) acc[m] += win[padw(o * s + t * d)] * kv;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* y = a.y + p.y_off + o0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int o = fast ? 8 * tid() + m : tid() + m * T;
+        if (o < cnt) y[o] = acc[m];
+    }
+}
+
+__device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
+    for (int i = tid(); i < n; i += T) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a) {
+    extern __shared__ C smem[];
+    __shared__ LineCtx lc[16];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0;   // work-queue head for the main launch
+    if (int(blockIdx.x) >= a.nkitems) return;
+    const BatchKItem it = a.kitems[blockIdx.x];
+    C* q = smem;
+    C* buf = smem + kTwSlots;
+    const int LC = batch_lc(it.cls);
+    stage_tw(q, a.tw + batch_tw_offset(it.cls), LC / 2);
+    __syncthreads();
+    switch (it.cls) {
+        case 0: kspec_item<256>(a, it, q, buf, lc); break;
+        case 1: kspec_item<512>(a, it, q, buf, lc); break;
+        case 2: kspec_item<1024>(a, it, q, buf, lc); break;
+        case 3: kspec_item<2048>(a, it, q, buf, lc); break;
+        default: kspec_item<4096>(a, it, q, buf, lc); break;
+    }
+}
+
+// Persistent: every workgroup takes items from the queue head until the
+// queue is empty (the only exit, reached by every workgroup).
+__global__ void __launch_bounds__(kBatchThreads) batch_main_kernel(BatchArgs a) {
+    extern __shared__ C smem[];
+    __shared__ LineCtx lc[16];
+    __shared__ int s_item;
+    C* q = smem;
+    C* buf = smem + kTwSlots;
+    float* win = reinterpret_cast<float*>(buf);
+    float* taps = win + kWinSlots;
+    int cur = -1;   // class whose twiddle table is in LDS
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int idx = s_item;
+        if (idx >= a.nitems) break;
+        const BatchItem it = a.items[idx];
+        if (it.kind == 0) {
+            direct_item(a, it, win, taps);
+        } else {
+            const int cls = it.kind - 1;
+            if (cls != cur) {
+                stage_tw(q, a.tw + batch_tw_offset(cls), batch_lc(cls) / 2);
+                cur = cls;
+                __syncthreads();
+            }
+            switch (cls) {
+                case 0: fft_item<256>(a, it, q, buf, lc); break;
+                case 1: fft_item<512>(a, it, q, buf, lc); break;
+                case 2: fft_item<1024>(a, it, q, buf, lc); break;
+                case 3: fft_item<2048>(a, it, q, buf, lc); break;
+                default: fft_item<4096>(a, it, q, buf, lc); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t batch_lds_bytes() { return kLdsBytes; }
+
+int batch_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_main_kernel, kBatchThreads, kLdsBytes) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+ndc_status launch_batch(const BatchArgs& a, int grid, LaunchContext& ctx, hipStream_t stream, bool profile,
+                        double kspec_bytes, double main_bytes, double main_flops) {
+    if (!lds_fits(ctx, reinterpret_cast<const void*>(batch_kspec_kernel), kLdsBytes) ||
+        !lds_fits(ctx, reinterpret_cast<const void*>(batch_main_kernel), kLdsBytes))
+        return NDC_ERR_UNSUPPORTED;
+    const int kgrid = a.nkitems > 0 ? a.nkitems : 1;
+    if (profile) ctx.kbegin("batch_kspec", kspec_bytes, 0);
+    hipLaunchKernelGGL(batch_kspec_kernel, dim3(kgrid), dim3(kBatchThreads), kLdsBytes, stream, a);
+    if (profile) ctx.kend();
+    if (hipGetLastError() != hipSuccess) {
+        ctx.err = "batch_kspec launch failed";
+        return NDC_ERR_DEVICE;
+    }
+    if (a.nitems > 0 && grid > 0) {
+        if (profile) ctx.kbegin("batch_main", main_bytes, main_flops);
+        hipLaunchKernelGGL(batch_main_kernel, dim3(grid), dim3(kBatchThreads), kLdsBytes, stream, a);
+        if (profile) ctx.kend();
+        if (hipGetLastError() != hipSuccess) {
+            ctx.err = "batch_main launch failed";
+            return NDC_ERR_DEVICE;
+        }
+    }
+    return NDC_OK;
+}
+
+}  // namespace ndc

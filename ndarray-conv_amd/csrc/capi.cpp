@@ -9,6 +9,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "batch.hpp"
 #include "launch.hpp"
 #include "ndconv.h"
 #include "plan.hpp"
@@ -252,6 +253,13 @@ ndc_status run(ndc_handle hh, const ndc_conv_desc* d, int op, const void* x, con
 }
 
 }  // namespace
+
+LaunchContext* handle_context(ndc_handle h) { return reinterpret_cast<Handle*>(h); }
+int handle_device(ndc_handle h) { return reinterpret_cast<Handle*>(h)->device; }
+void handle_set_last(ndc_handle h, const std::string& msg) {
+    if (h) reinterpret_cast<Handle*>(h)->last = msg;
+}
+
 }  // namespace ndc
 
 using ndc::Handle;
@@ -268,6 +276,7 @@ int64_t ndc_struct_size(int32_t which) {
         case 3: return int64_t(sizeof(ndc_padding_mode));
         case 4: return int64_t(sizeof(ndc_border));
         case 5: return int64_t(sizeof(ndc_value));
+        case 6: return int64_t(sizeof(ndc_batch_desc));
         default: return -1;
     }
 }

@@ -28,10 +28,13 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "fft_bfly.hpp"
 #include "launch.hpp"
 
 namespace ndc {
 namespace {
+
+using namespace fftb;
 
 constexpr int kMaxThreads = 1024;
 constexpr int kMaxStages = 24;
@@ -60,118 +63,6 @@ struct LinePlan {
     int base[kMaxStages];   // stage twiddles W_{Ns*P}^(r*k) at base + k*(P-1) + r-1
     int wp[kMaxStages];     // radix-P roots W_P^m at wp + m (stages without a fixed butterfly)
 };
-
-// ---------------------------------------------------------------------------
-// Small DFT butterflies.  Forward uses W = exp(-2*pi*i/P); inverse the
-// conjugate, unnormalised (rustfft's convention).
-// ---------------------------------------------------------------------------
-
-template <typename C>
-__device__ __forceinline__ void bfly2(C* v) {
-    C a = v[0];
-    v[0] = cadd(a, v[1]);
-    v[1] = csub(a, v[1]);
-}
-
-template <typename C>
-__device__ __forceinline__ void bfly4(C* v, bool inv) {
-    C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-    C r = inv ? cmul_i(d13) : cmul_negi(d13);
-    v[0] = cadd(s02, s13);
-    v[2] = csub(s02, s13);
-    v[1] = cadd(d02, r);
-    v[3] = csub(d02, r);
-}
-
-template <typename C>
-__device__ __forceinline__ void bfly8(C* v, bool inv) {
-    using R = decltype(C{}.x);
-    const R h = R(0.70710678118654757);
-    C a[4], b[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        a[r] = cadd(v[r], v[r + 4]);
-        b[r] = csub(v[r], v[r + 4]);
-    }
-    // b[r] *= W_8^r
-    if (!inv) {
-        b[1] = cmake<C>(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
-        b[2] = cmul_negi(b[2]);
-        b[3] = cmake<C>(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
-    } else {
-        b[1] = cmake<C>(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
-        b[2] = cmul_i(b[2]);
-        b[3] = cmake<C>(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
-    }
-    bfly4(a, inv);
-    bfly4(b, inv);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        v[2 * q] = a[q];
-        v[2 * q + 1] = b[q];
-    }
-}
-
-// cos/sin(2*pi*m/P), m = 0 .. (P-1)/2
-template <int P> struct OddConst;
-template <> struct OddConst<3> {
-    static constexpr double c[2] = {1.0, -0.5};
-    static constexpr double s[2] = {0.0, 0.86602540378443865};
-};
-template <> struct OddConst<5> {
-    static constexpr double c[3] = {1.0, 0.30901699437494745, -0.80901699437494745};
-    static constexpr double s[3] = {0.0, 0.95105651629515353, 0.58778525229247314};
-};
-template <> struct OddConst<7> {
-    static constexpr double c[4] = {1.0, 0.62348980185873359, -0.22252093395631434, -0.90096886790241915};
-    static constexpr double s[4] = {0.0, 0.78183148246802980, 0.97492791218182362, 0.43388373911755812};
-};
-
-// y[k] = v0 + sum_n a_n cos(th n k) -/+ i sum_n b_n sin(th n k), with
-// a_n = v[n] + v[P-n], b_n = v[n] - v[P-n].
-template <int P, typename C>
-__device__ __forceinline__ void bfly_odd(C* v, bool inv) {
-    using R = decltype(C{}.x);
-    constexpr int H = (P - 1) / 2;
-    C a[H + 1], b[H + 1];
-#pragma unroll
-    for (int n = 1; n <= H; ++n) {
-        a[n] = cadd(v[n], v[P - n]);
-        b[n] = csub(v[n], v[P - n]);
-    }
-    C y0 = v[0];
-#pragma unroll
-    for (int n = 1; n <= H; ++n) y0 = cadd(y0, a[n]);
-    C out[P];
-    out[0] = y0;
-#pragma unroll
-    for (int k = 1; k <= H; ++k) {
-        C A = v[0], B = cmake<C>(R(0), R(0));
-#pragma unroll
-        for (int n = 1; n <= H; ++n) {
-            const int m = (n * k) % P;
-            const R cs = R(OddConst<P>::c[m <= H ? m : P - m]);
-            const R sn = m <= H ? R(OddConst<P>::s[m]) : R(-OddConst<P>::s[P - m]);
-            A = cadd(A, cscale(a[n], cs));
-            B = cadd(B, cscale(b[n], sn));
-        }
-        // forward: -i*B ; inverse: +i*B
-        const C t = inv ? cmul_i(B) : cmul_negi(B);
-        out[k] = cadd(A, t);
-        out[P - k] = csub(A, t);
-    }
-#pragma unroll
-    for (int k = 0; k < P; ++k) v[k] = out[k];
-}
-
-template <int P, typename C>
-__device__ __forceinline__ void bfly(C* v, bool inv) {
-    if constexpr (P == 2) bfly2(v);
-    else if constexpr (P == 4) bfly4(v, inv);
-    else if constexpr (P == 8) bfly8(v, inv);
-    else bfly_odd<P>(v, inv);
-}
 
 // ---------------------------------------------------------------------------
 // Stockham stages over lines in LDS: element (line b, index t) at b*ls + t*es.

@@ -218,8 +218,11 @@ __global__ void __launch_bounds__(kThreads) mfma_conv_kernel(MfmaArgs a) {
                         floatx16 c;
 #pragma unroll
                         for (int i = 0; i < 16; ++i) {
-                            const int v = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, acc[s][i]), 0x138,
-                                                                      0xf, 0xf, true);
+                            // copy the element out first: a bit cast of the vector
+                            // element lvalue reads element 0
+                            const float ai = acc[s][i];
+                            const int v = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, ai), 0x138, 0xf, 0xf,
+                                                                      true);
                             c[i] = m == 0 ? 0.0f : __builtin_bit_cast(float, v);
                         }
 #pragma unroll

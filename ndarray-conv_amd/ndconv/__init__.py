@@ -12,6 +12,8 @@ The names, argument meaning and error behaviour follow the reference crate
     BorderType.Zeros/Const/Reflect/Replicate/Circular                             lib.rs:129-143
     with_dilation(kernel, d), reverse(k), no_reverse(k)                           dilation/mod.rs:103-189
     DataShape / KernelShape / MismatchShape errors                                lib.rs:145-159
+    BatchPlan / conv_fft_batch: many rank-1 pairs, conv_fft semantics per pair,
+    one persistent launch (the reference loops over conv_fft calls)
 
 `data` and `kernel` are numpy arrays (host memory, copied to and from the
 GPU inside the call) or torch tensors already on the GPU (HBM-resident; the
@@ -30,7 +32,7 @@ from . import _lib as C
 __all__ = ["ConvMode", "PaddingMode", "BorderType", "KernelWithDilation", "with_dilation", "reverse",
            "no_reverse", "conv", "conv_fft", "conv_fft_with_processor", "conv_fft_par", "get_fft_processor",
            "FftProcessor", "plan", "border_map", "Error", "DataShape", "KernelShape", "MismatchShape",
-           "InvalidArgument", "Unsupported", "DeviceError"]
+           "InvalidArgument", "Unsupported", "DeviceError", "BatchPlan", "conv_fft_batch", "batch_border_map"]
 
 
 # ---------------------------------------------------------------------------
@@ -481,3 +483,146 @@ def conv_fft_par(data, kernel, conv_mode, padding_mode):
     """ConvFFTExt::conv_fft_par (conv_fft/mod.rs:166-171): the same transform;
     the GPU grid replaces rayon."""
     return _run(C.OP_CONV_FFT, data, kernel, conv_mode, padding_mode)
+
+
+# ---------------------------------------------------------------------------
+# Batches of independent rank-1 pairs (one conv_fft call per pair in the
+# reference; one persistent launch here)
+# ---------------------------------------------------------------------------
+
+_BATCH_ALGOS = {"auto": C.BATCH_AUTO, "fft": C.BATCH_FFT, "direct": C.BATCH_DIRECT, "plain": C.BATCH_PLAIN}
+
+
+def _batch_desc(x_lens, k_lens, conv_mode, padding_mode, dilation, reverse_flag, algo, host):
+    xl = np.ascontiguousarray(x_lens, dtype=np.int64)
+    kl = np.ascontiguousarray(k_lens, dtype=np.int64)
+    if xl.ndim != 1 or xl.shape != kl.shape:
+        raise InvalidArgument("x_lens and k_lens must be 1-D of equal length")
+    tmpl = _desc_for_shapes((1,), (1,), conv_mode, padding_mode, dilation, reverse_flag)
+    d = C.BatchDesc()
+    d.dtype = C.F32
+    if algo not in _BATCH_ALGOS:
+        raise InvalidArgument(f"unknown batch algorithm {algo!r}")
+    d.algo = _BATCH_ALGOS[algo]
+    d.count = int(xl.shape[0])
+    d.x_len = xl.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    d.k_len = kl.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    d.dilation = int(dilation)
+    d.reverse = 1 if reverse_flag else 0
+    d.flags = C.FLAG_HOST_MEMORY if host else 0
+    d.conv_mode = tmpl.conv_mode
+    d.padding_mode = tmpl.padding_mode
+    return d, xl, kl
+
+
+class BatchPlan:
+    """Many independent rank-1 f32 signal/kernel pairs of uneven lengths.
+
+    Each pair has the semantics of one conv_fft call (conv_fft/mod.rs:113-118)
+    with the shared ConvMode, PaddingMode, dilation and reverse flag.  Signals,
+    kernels and outputs are packed back to back in pair order.  algo: "auto"
+    (device-measured cost model picks direct or overlap-save FFT per pair),
+    "fft", "direct", or "plain" (one single-problem conv_fft per pair).
+    host=True takes numpy arrays; otherwise torch GPU tensors, ordered on
+    torch's current stream."""
+
+    def __init__(self, x_lens, k_lens, conv_mode, padding_mode, dilation=1, reverse_flag=True, algo="auto",
+                 host=False, processor=None):
+        self._proc = processor or _default_processor()
+        self.host = bool(host)
+        lib = C.lib()
+        h = self._proc.handle
+        d, self.x_lens, self.k_lens = _batch_desc(x_lens, k_lens, conv_mode, padding_mode, dilation, reverse_flag,
+                                                  algo, host)
+        b = ctypes.c_void_p()
+        _check(lib.ndc_batch_create(h, ctypes.byref(d), ctypes.byref(b)), h)
+        self._b = b
+        n = int(self.x_lens.shape[0])
+        self.y_lens = np.zeros(n, dtype=np.int64)
+        xt, kt, yt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib.ndc_batch_out_lens(b, self.y_lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                      ctypes.byref(xt), ctypes.byref(kt), ctypes.byref(yt)), h)
+        self.x_total, self.k_total, self.y_total = xt.value, kt.value, yt.value
+
+    def _offsets(self, lens):
+        o = np.zeros(len(lens) + 1, dtype=np.int64)
+        np.cumsum(lens, out=o[1:])
+        return o
+
+    def run(self, x, k, y=None, graph=False):
+        """Convolves the packed batch; returns the packed outputs."""
+        lib = C.lib()
+        h = self._proc.handle
+        fn = lib.ndc_batch_run_graph if graph else lib.ndc_batch_run
+        if self.host:
+            x = np.ascontiguousarray(x, dtype=np.float32)
+            k = np.ascontiguousarray(k, dtype=np.float32)
+            if x.size != self.x_total or k.size != self.k_total:
+                raise InvalidArgument("packed sizes differ from the plan")
+            if y is None:
+                y = np.empty(self.y_total, dtype=np.float32)
+            _check(lib.ndc_reset_stream(h), h)
+            _check(fn(self._b, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(k.ctypes.data),
+                      ctypes.c_void_p(y.ctypes.data)), h)
+            return y
+        import torch
+        if not (_is_torch(x) and _is_torch(k)) or not x.is_cuda or not k.is_cuda:
+            raise InvalidArgument("a device batch takes torch GPU tensors")
+        if x.dtype != torch.float32 or k.dtype != torch.float32 or not x.is_contiguous() or not k.is_contiguous():
+            raise InvalidArgument("packed inputs must be contiguous float32")
+        if x.numel() != self.x_total or k.numel() != self.k_total:
+            raise InvalidArgument("packed sizes differ from the plan")
+        if y is None:
+            y = torch.empty(self.y_total, dtype=torch.float32, device=x.device)
+        _check(lib.ndc_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)), h)
+        _check(fn(self._b, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(k.data_ptr()),
+                  ctypes.c_void_p(y.data_ptr())), h)
+        return y
+
+    def split(self, y):
+        """The per-pair outputs of a packed result."""
+        o = self._offsets(self.y_lens)
+        return [y[o[i]:o[i + 1]] for i in range(len(self.y_lens))]
+
+    def report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(C.lib().ndc_batch_report(self._b, buf, len(buf)), self._proc.handle)
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "_b", None):
+            C.lib().ndc_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def conv_fft_batch(xs, ks, conv_mode, padding_mode, dilation=1, reverse_flag=True, algo="auto"):
+    """conv_fft of every (xs[i], ks[i]) pair (rank-1 f32 numpy arrays), as one
+    batch; returns the list of outputs."""
+    xs = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in xs]
+    ks = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in ks]
+    bp = BatchPlan([a.size for a in xs], [a.size for a in ks], conv_mode, padding_mode, dilation, reverse_flag,
+                   algo, host=True)
+    try:
+        x = np.concatenate(xs) if xs else np.zeros(0, np.float32)
+        k = np.concatenate(ks) if ks else np.zeros(0, np.float32)
+        return [a.copy() for a in bp.split(bp.run(x, k))]
+    finally:
+        bp.close()
+
+
+def batch_border_map(x_lens, k_lens, pair, conv_mode, padding_mode, dilation=1):
+    """Closed-form border map of one pair of a batch (see ndc_batch_border_map)."""
+    d, xl, kl = _batch_desc(x_lens, k_lens, conv_mode, padding_mode, dilation, True, "auto", False)
+    info = plan((int(xl[pair]),), (int(kl[pair]),), conv_mode, padding_mode, dilation)
+    n = info["padded"][0]
+    out = np.zeros(n, dtype=np.int32)
+    _check(C.lib().ndc_batch_border_map(ctypes.byref(d), int(pair),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n))
+    return out

@@ -24,6 +24,8 @@ ALGO_AUTO, ALGO_DIRECT, ALGO_MFMA_BF16, ALGO_MFMA_BF16X3 = range(4)
 FLAG_HOST_MEMORY, FLAG_X_STRIDED, FLAG_K_STRIDED = 1, 2, 4
 # ndc_op
 OP_CONV, OP_CONV_FFT = 0, 1
+# ndc_batch_algo
+BATCH_AUTO, BATCH_FFT, BATCH_DIRECT, BATCH_PLAIN = range(4)
 
 I64A = ctypes.c_int64 * MAX_RANK
 
@@ -52,6 +54,13 @@ class ConvDesc(ctypes.Structure):
                 ("dilation", I64A), ("reverse", ctypes.c_int32), ("algo", ctypes.c_int32),
                 ("conv_mode", ConvModeC), ("padding_mode", PaddingModeC),
                 ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("algo", ctypes.c_int32), ("count", ctypes.c_int64),
+                ("x_len", ctypes.POINTER(ctypes.c_int64)), ("k_len", ctypes.POINTER(ctypes.c_int64)),
+                ("dilation", ctypes.c_int64), ("reverse", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("conv_mode", ConvModeC), ("padding_mode", PaddingModeC)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -103,5 +112,22 @@ def lib():
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [H, ctypes.POINTER(ConvDesc), P, P, P]
+    B = ctypes.c_void_p
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    L.ndc_batch_create.restype = ctypes.c_int
+    L.ndc_batch_create.argtypes = [H, ctypes.POINTER(BatchDesc), ctypes.POINTER(B)]
+    L.ndc_batch_destroy.restype = ctypes.c_int
+    L.ndc_batch_destroy.argtypes = [B]
+    L.ndc_batch_out_lens.restype = ctypes.c_int
+    L.ndc_batch_out_lens.argtypes = [B, I64P, I64P, I64P, I64P]
+    for name in ("ndc_batch_run", "ndc_batch_run_graph"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [B, P, P, P]
+    L.ndc_batch_report.restype = ctypes.c_int
+    L.ndc_batch_report.argtypes = [B, ctypes.c_char_p, ctypes.c_int64]
+    L.ndc_batch_border_map.restype = ctypes.c_int
+    L.ndc_batch_border_map.argtypes = [ctypes.POINTER(BatchDesc), ctypes.c_int64, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.c_int64]
     _lib = L
     return L

@@ -36,7 +36,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_binding():
     lib = C.lib()
-    for which, st in enumerate([C.ConvDesc, C.PlanInfo, C.ConvModeC, C.PaddingModeC, C.Border, C.Value]):
+    for which, st in enumerate([C.ConvDesc, C.PlanInfo, C.ConvModeC, C.PaddingModeC, C.Border, C.Value,
+                                C.BatchDesc]):
         assert lib.ndc_struct_size(which) == ctypes.sizeof(st), st.__name__
     assert lib.ndc_struct_size(99) == -1
 

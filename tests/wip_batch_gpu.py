@@ -1,0 +1,170 @@
+"""The batched rank-1 path on the GPU: every pair must match the FP64 oracle's
+conv_fft of that pair within the 1e-5 normwise relative error of the plain
+FP32 path, whichever form (direct, overlap-save FFT, autotuned mix, or the
+unbatched per-pair path) computes it."""
+import numpy as np
+import pytest
+
+from modes import conv_mode, padding_mode, random_border
+from oracle import ndconv_oracle as O
+
+import ndconv as nc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+ALGOS = ["auto", "fft", "direct", "plain"]
+
+
+def oracle_pairs(xs, ks, mode, pm, dil, rev):
+    return [O.conv_fft(x.astype(np.float64), k.astype(np.float64), mode, pm, dil, rev) for x, k in zip(xs, ks)]
+
+
+def check(xs, ks, mode, pm, dil=1, rev=True, algos=ALGOS, tol=TOL):
+    refs = oracle_pairs(xs, ks, mode, pm, dil, rev)   # the oracle runs first: it raises on invalid input
+    worst = {}
+    for algo in algos:
+        got = nc.conv_fft_batch(xs, ks, conv_mode(mode), padding_mode(pm), dil, rev, algo)
+        assert len(got) == len(refs)
+        w = 0.0
+        for i, (g, r) in enumerate(zip(got, refs)):
+            assert g.shape == r.shape, (algo, i)
+            e = O.relerr(g, r)[0]
+            assert e <= tol, (algo, i, len(xs[i]), len(ks[i]), e)
+            w = max(w, e)
+        worst[algo] = w
+    return worst
+
+
+@pytest.mark.parametrize("pm", ["circular", "zeros", "replicate", "reflect", ("const", 0.75)])
+def test_batch_same_mode_borders(gpu, pm):
+    rng = np.random.default_rng(31)
+    n = rng.integers(1, 3000, 40)
+    k = rng.integers(1, 500, 40)
+    if pm == "reflect":
+        k = np.minimum(k, 2 * n - 1)   # reflect needs pads shorter than the signal
+    xs = [rng.uniform(-1, 1, int(a)).astype(np.float32) for a in n]
+    ks = [rng.uniform(-1, 1, int(b)).astype(np.float32) for b in k]
+    print(pm, check(xs, ks, "same", pm))
+
+
+def test_batch_front_pad_longer_than_signal(gpu):
+    # Same + Circular with kernels much longer than the signal: the front pad
+    # wraps once and the cells before p0 - n stay 0
+    rng = np.random.default_rng(32)
+    n = rng.integers(1, 40, 30)
+    k = rng.integers(60, 1023, 30)
+    xs = [rng.uniform(-1, 1, int(a)).astype(np.float32) for a in n]
+    ks = [rng.uniform(-1, 1, int(b)).astype(np.float32) for b in k]
+    print(check(xs, ks, "same", "circular"))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_batch_random_modes(gpu, seed):
+    rng = np.random.default_rng(100 + seed)
+    for case in range(4):
+        count = int(rng.integers(1, 25))
+        dil = int(rng.integers(1, 4))
+        rev = bool(rng.integers(0, 2))
+        s = int(rng.integers(1, 4))
+        kmax = int(rng.integers(2, 300))
+        n = rng.integers(1, 4000, count)
+        k = rng.integers(1, kmax, count)
+        choice = int(rng.integers(0, 5))
+        if choice == 0:
+            mode = "full"
+        elif choice == 1:
+            mode = "valid"
+            n = np.maximum(n, (k - 1) * dil + 1)
+        elif choice == 2:
+            mode = "same"
+        elif choice == 3:
+            mode = ("custom", [int(rng.integers(0, 200))], [s])
+        else:
+            mode = ("explicit", [[int(rng.integers(0, 300)), int(rng.integers(0, 300))]], [s])
+        pm = ("explicit", [[random_border(rng, 0.5), random_border(rng, -0.25)]])
+        xs = [rng.uniform(-1, 1, int(a)).astype(np.float32) for a in n]
+        ks = [rng.uniform(-1, 1, int(b)).astype(np.float32) for b in k]
+        try:
+            oracle_pairs(xs, ks, mode, pm, dil, rev)
+        except O.OracleError as e:
+            with pytest.raises(getattr(nc, e.kind)):
+                nc.conv_fft_batch(xs, ks, conv_mode(mode), padding_mode(pm), dil, rev)
+            continue
+        print(seed, case, mode, check(xs, ks, mode, pm, dil, rev))
+
+
+def test_batch_long_signals_and_kernels(gpu):
+    # the largest FFT block (8192 samples), kernels up to 1023 taps and
+    # signals needing hundreds of blocks
+    rng = np.random.default_rng(33)
+    n = np.array([262144, 100000, 65537, 8191, 300, 20000])
+    k = np.array([1023, 1000, 512, 700, 1023, 3])
+    xs = [rng.uniform(-1, 1, int(a)).astype(np.float32) for a in n]
+    ks = [rng.uniform(-1, 1, int(b)).astype(np.float32) for b in k]
+    print(check(xs, ks, "same", "circular", algos=["auto", "fft", "direct"]))
+
+
+def test_batch_errors_follow_the_reference(gpu):
+    x = [np.ones(10, np.float32), np.ones(0, np.float32)]
+    k = [np.ones(3, np.float32), np.ones(3, np.float32)]
+    with pytest.raises(nc.DataShape):
+        nc.conv_fft_batch(x, k, nc.ConvMode.Same, nc.PaddingMode.Zeros)
+    x = [np.ones(10, np.float32), np.ones(5, np.float32)]
+    k = [np.ones(3, np.float32), np.ones(0, np.float32)]
+    with pytest.raises(nc.DataShape):          # conv_fft reports an empty kernel as DataShape
+        nc.conv_fft_batch(x, k, nc.ConvMode.Same, nc.PaddingMode.Zeros)
+    k = [np.ones(3, np.float32), np.ones(20, np.float32)]
+    with pytest.raises(nc.MismatchShape):
+        nc.conv_fft_batch(x, k, nc.ConvMode.Valid, nc.PaddingMode.Zeros)
+    with pytest.raises(nc.InvalidArgument):    # reflect pad beyond the signal panics in the reference
+        nc.conv_fft_batch(x, k, nc.ConvMode.Same, nc.PaddingMode.Reflect)
+    assert nc.conv_fft_batch([], [], nc.ConvMode.Same, nc.PaddingMode.Zeros) == []
+
+
+def test_uneven_batch_full_size(gpu):
+    """The 4096-pair uneven batch (seed 5): a sample of pairs against the FP64
+    oracle, every pair across the forms, graph replays against eager runs."""
+    import torch
+    from bench import uneven_batch
+    n, k, x, kv = uneven_batch(5, 4096)
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(x).to(dev)
+    kt = torch.from_numpy(kv).to(dev)
+    outs = {}
+    for algo in ["auto", "fft", "direct"]:
+        bp = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo=algo)
+        y = bp.run(xt, kt)
+        torch.cuda.synchronize()
+        outs[algo] = y.cpu().numpy()
+        if algo == "auto":
+            rep = bp.report()
+            print({kk: rep[kk] for kk in ("items", "lines", "grid", "costs")})
+            for r in range(3):
+                yg = bp.run(xt, kt, graph=True)
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(yg.cpu().numpy(), outs["auto"], err_msg=f"replay {r}")
+                bp.run(xt, kt)   # eager work between replays
+            torch.cuda.synchronize()
+            ylens, yoff = bp.y_lens, np.concatenate([[0], np.cumsum(bp.y_lens)])
+        bp.close()
+    xo = np.concatenate([[0], np.cumsum(n)])
+    ko = np.concatenate([[0], np.cumsum(k)])
+    rng = np.random.default_rng(0)
+    sample = rng.choice(len(n), 48, replace=False)
+    worst = 0.0
+    for i in sample:
+        ref = O.conv_fft(x[xo[i]:xo[i + 1]].astype(np.float64), kv[ko[i]:ko[i + 1]].astype(np.float64), "same",
+                         "circular")
+        assert len(ref) == ylens[i]
+        for algo, y in outs.items():
+            e = O.relerr(y[yoff[i]:yoff[i + 1]], ref)[0]
+            assert e <= TOL, (algo, int(i), int(n[i]), int(k[i]), e)
+            worst = max(worst, e)
+    # every pair: the three forms agree to the tolerance
+    for i in range(len(n)):
+        a = outs["auto"][yoff[i]:yoff[i + 1]]
+        for algo in ("fft", "direct"):
+            e = O.relerr(outs[algo][yoff[i]:yoff[i + 1]], a)[0]
+            assert e <= 2 * TOL, (algo, i, e)
+    print(f"uneven batch: worst sampled normwise rel err {worst:.3e}")
