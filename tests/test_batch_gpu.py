@@ -117,7 +117,11 @@ def test_batch_errors_follow_the_reference(gpu):
     k = [np.ones(3, np.float32), np.ones(20, np.float32)]
     with pytest.raises(nc.MismatchShape):
         nc.conv_fft_batch(x, k, nc.ConvMode.Valid, nc.PaddingMode.Zeros)
-    with pytest.raises(nc.InvalidArgument):    # reflect pad beyond the signal panics in the reference
+    # reflect: a front pad of at least n + back pad panics in the reference
+    x = [np.ones(10, np.float32), np.ones(1, np.float32)]
+    with pytest.raises(O.OracleError):
+        O.conv_fft(x[1], k[1], "same", "reflect")
+    with pytest.raises(nc.InvalidArgument):
         nc.conv_fft_batch(x, k, nc.ConvMode.Same, nc.PaddingMode.Reflect)
     assert nc.conv_fft_batch([], [], nc.ConvMode.Same, nc.PaddingMode.Zeros) == []
 
