@@ -93,6 +93,43 @@ def make_inputs(w):
     return rng.standard_normal(w["x"]).astype(dt), rng.standard_normal(w["k"]).astype(dt)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` in `workload`, measured with rocprofv3
+    --pmc FETCH_SIZE and WRITE_SIZE and scaled by what the same counters read
+    for a plain device copy of known size (the `copy` workload); None when
+    that measurement has not been recorded."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def run_copy(args):
+    """A plain device-to-device copy of 512 MiB per step: the reference
+    kernel for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 128 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    print(json.dumps({"workload": "copy", "bytes_read_per_step": 4 * n, "bytes_written_per_step": 4 * n,
+                      "ms_per_step": ms, "GB/s": 8 * n / (ms * 1e-3) / 1e9}))
+
+
 def cpu_baseline(w, x, k, budget_s=12.0):
     """The oracle's f32 restatement (scipy pocketfft) on this host's cores."""
     if w["cpu"] is None:
@@ -202,8 +239,8 @@ def run_batch(args):
     bpl = dom["bytes"] / dom["launches"]
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom_name, "avg_ms": avg_ms,
-            "bytes_per_launch": bpl}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch1d", dom_name), "kernel": dom_name,
+            "avg_ms": avg_ms, "bytes_per_launch": bpl}
     kernels = {nm: {"avg_ms": v["total_ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
                     "GB/s": (v["bytes"] / v["launches"]) / (v["total_ms"] / v["launches"] * 1e-3) / 1e9}
                for nm, v in prof.items()}
@@ -252,6 +289,7 @@ def main():
     ap.add_argument("--workload", default="fft2d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step")
+    ap.add_argument("--algo", default="auto", help="direct workloads: auto (exact FP32/integer), bf16, bf16x3")
     ap.add_argument("--batch-algo", default="auto", help="uneven batch: auto, fft, direct")
     ap.add_argument("--plain-steps", type=int, default=2, help="uneven batch: steps of the unbatched path")
     args = ap.parse_args()
@@ -260,6 +298,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch1d":
         run_batch(args)
+        return
+    if args.workload == "copy":
+        run_copy(args)
         return
 
     import torch
@@ -277,6 +318,7 @@ def main():
     d = C.ConvDesc()
     kwd = nc.with_dilation(kt, w["dil"])
     nc._fill_desc(d, xt, kwd, w["mode"], w["pad"], nc._NP_DTYPE[np.dtype(w["dtype"])], host=False)
+    d.algo = nc._ALGOS[args.algo]
     op = C.OP_CONV_FFT if w["op"] == "fft" else C.OP_CONV
     info = C.PlanInfo()
     nc._check(lib.ndc_plan(ctypes.byref(d), op, ctypes.byref(info)))
@@ -323,11 +365,18 @@ def main():
     if w["op"] == "fft":
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+                "frac": achieved / HBM_PEAK_GBS}
+    elif args.algo in ("bf16", "bf16x3"):
+        # FP32-equivalent algorithmic flops (2 per output and tap) against the
+        # dense BF16 matrix-core peak; BF16x3 issues three products per tap
+        achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / BF16_PEAK_TFLOPS, "products_per_tap": 3 if args.algo == "bf16x3" else 1}
     else:
         achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+                "frac": achieved / FP32_PEAK_TFLOPS}
+    roof["traffic"] = measured_traffic(args.workload, dom_name)
     roof["kernel"] = dom_name
     roof["avg_ms"] = avg_ms
     roof["bytes_per_launch"] = bytes_per_launch
@@ -365,7 +414,7 @@ def main():
             "vs_baseline": None, "dtype": {np.dtype(np.float32): "f32", np.dtype(np.complex64): "c64",
                                            np.dtype(np.int32): "i32"}[np.dtype(w["dtype"])],
             "data": "synthetic", "config": {"workload": w["name"], "outputs_per_step": outputs,
-                                            "seed": w["seed"]},
+                                            "seed": w["seed"], "algo": args.algo},
             "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall}
     if graph_value is not None:
         line["graph_replay_value"] = graph_value

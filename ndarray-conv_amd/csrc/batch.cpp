@@ -166,10 +166,8 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
             S.ks_total += LC + 1;
             cls_kpairs[c].push_back(int32_t(i));
             S.kspec_bytes += 4.0 * double(ps.k) + 8.0 * double(LC + 1);
-            for (int64_t o = 0; o < ps.out; o += bp.per) {
-                cls_lines[c].push_back(BatchLine{int32_t(i), int32_t(o)});
-                S.main_bytes += 8.0 * double(LC + 1);   // the kernel spectrum, read per block
-            }
+            S.main_bytes += 8.0 * double(LC + 1);   // the kernel spectrum, read once (its blocks share it in cache)
+            for (int64_t o = 0; o < ps.out; o += bp.per) cls_lines[c].push_back(BatchLine{int32_t(i), int32_t(o)});
         }
     }
     for (int c = 0; c < kBatchClasses; ++c) {
