@@ -246,6 +246,28 @@ def run_batch(args):
                for nm, v in prof.items()}
     report = bp.report()
 
+    # the FP32-only autotuned form next to it
+    fp32 = None
+    if args.batch_algo != "auto":
+        bf = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo="auto", processor=proc)
+        for _ in range(args.warmup):
+            bf.run(xt, kt, yt, graph=True)
+        torch.cuda.synchronize()
+        f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f0.record(stream)
+        for _ in range(args.steps):
+            bf.run(xt, kt, yt, graph=True)
+        f1.record(stream)
+        torch.cuda.synchronize()
+        fp32 = f0.elapsed_time(f1) / args.steps
+        yf = yt.cpu().numpy()
+        worst_fp32 = 0.0
+        for i in np.random.default_rng(1).choice(len(n), 16, replace=False):
+            ref = O.conv_fft(x[xo[i]:xo[i + 1]].astype(np.float64), kv[ko[i]:ko[i + 1]].astype(np.float64), "same",
+                             "circular")
+            worst_fp32 = max(worst_fp32, O.relerr(yf[yo[i]:yo[i + 1]], ref)[0])
+        bf.close()
+
     # the unbatched path: one single-problem conv_fft per pair
     plain = None
     if args.plain_steps > 0:
@@ -274,6 +296,10 @@ def run_batch(args):
             "max_rel_err_sampled": worst, "plan_s": plan_s,
             "schedule": {kk: report[kk] for kk in ("items", "lines", "kernel_spectra", "grid", "blocks_per_cu",
                                                    "costs")}}
+    if fp32 is not None:
+        line["fp32_auto_ms_per_step"] = fp32
+        line["fp32_auto_value"] = outputs / (fp32 * 1e-3)
+        line["fp32_auto_max_rel_err_sampled"] = worst_fp32
     if plain is not None:
         line["plain_ms_per_step"] = plain
         line["plain_value"] = outputs / (plain * 1e-3)
@@ -290,7 +316,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step")
     ap.add_argument("--algo", default="auto", help="direct workloads: auto (exact FP32/integer), bf16, bf16x3")
-    ap.add_argument("--batch-algo", default="auto", help="uneven batch: auto, fft, direct")
+    ap.add_argument("--batch-algo", default="auto_bf16x3",
+                    help="uneven batch: auto_bf16x3 (autotuned BF16x3/FP32 mix), auto (FP32 only), fft, direct, bf16x3")
     ap.add_argument("--plain-steps", type=int, default=2, help="uneven batch: steps of the unbatched path")
     args = ap.parse_args()
     if args.gpus != 1:

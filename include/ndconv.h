@@ -219,13 +219,21 @@ ndc_status ndc_profile_report(ndc_handle h, char* buf, int64_t len);
  * ------------------------------------------------------------------------- */
 
 typedef enum ndc_batch_algo {
-    NDC_BATCH_AUTO = 0,     /* per pair, the cheaper of direct and overlap-save FFT by the
-                             * device-measured cost model (the autotuner) */
+    NDC_BATCH_AUTO = 0,     /* per pair, the cheaper of FP32 direct and FP32 overlap-save FFT
+                             * by the device-measured cost model (the autotuner) */
     NDC_BATCH_FFT = 1,      /* overlap-save FFT for every pair whose dilated kernel fits
                              * the largest block (8192 samples), direct otherwise */
     NDC_BATCH_DIRECT = 2,   /* direct FP32 for every pair */
-    NDC_BATCH_PLAIN = 3     /* one single-problem conv_fft launch sequence per pair (the
+    NDC_BATCH_PLAIN = 3,    /* one single-problem conv_fft launch sequence per pair (the
                              * unbatched path, for comparison); run_graph runs it eagerly */
+    NDC_BATCH_MFMA = 4,     /* direct on the matrix cores in split BF16x3 (FP32 accumulation)
+                             * for every pair with stride 1 and a dilated kernel of at most
+                             * 1792 taps; the others as NDC_BATCH_AUTO */
+    NDC_BATCH_AUTO_BF16X3 = 5 /* the autotuner's choice among FP32 direct, BF16x3 direct and
+                               * FP32 FFT.  BF16x3 products carry about 16 significant bits:
+                               * on sums of many unrelated terms the normwise error stays near
+                               * 5e-6, but an output dominated by a few input samples can reach
+                               * 3 * 2^-16 relative */
 } ndc_batch_algo;
 
 typedef struct ndc_batch_desc {

@@ -54,7 +54,8 @@ struct DevMem {
 // Throughput costs (ms per item with the device full of workgroups).
 struct CostModel {
     double fft_item[kBatchClasses];
-    double dir_a, dir_b;   // direct item: dir_a + dir_b * outputs * taps
+    double dir_a, dir_b;     // FP32 direct item: dir_a + dir_b * outputs * taps
+    double mfma_a, mfma_b;   // BF16x3 matrix-core item: mfma_a + mfma_b * K-blocks
     bool measured = false;
 };
 
@@ -64,6 +65,8 @@ CostModel default_costs() {
     for (int i = 0; i < kBatchClasses; ++i) c.fft_item[i] = 1.2e-5;
     c.dir_a = 2e-6;
     c.dir_b = 6e-11;
+    c.mfma_a = 4e-6;
+    c.mfma_b = 4e-7;
     return c;
 }
 
@@ -97,9 +100,22 @@ double fft_cost(const CostModel& cm, const PairShape& ps, int s, int cls) {
     return (double(lines) + 0.5) / double(batch_lines(cls)) * cm.fft_item[cls];
 }
 
-// Method per pair: -1 direct, else FFT class.
+constexpr int kMethodDirect = -1;   // FP32 direct items
+constexpr int kMethodMfma = -2;     // BF16x3 matrix-core items
+
+int mfma_nkb(const PairShape& ps) { return int((ps.kdil + 31 + 15) / 16); }
+
+bool mfma_eligible(const PairShape& ps, int s) { return s == 1 && ps.kdil <= kBatchMfmaMaxKdil; }
+
+double mfma_cost(const CostModel& cm, const PairShape& ps, int s) {
+    if (!mfma_eligible(ps, s)) return INFINITY;
+    const int64_t nt = (ps.out + kBatchMfmaTile - 1) / kBatchMfmaTile;
+    return double(nt) * (cm.mfma_a + cm.mfma_b * double(mfma_nkb(ps)));
+}
+
+// Method per pair: kMethodDirect, kMethodMfma, or an FFT class.
 int choose(const CostModel& cm, const PairShape& ps, int s, int algo) {
-    int best_cls = -1;
+    int best_cls = kMethodDirect;
     double best = INFINITY;
     for (int c = 0; c < kBatchClasses; ++c) {
         const double v = fft_cost(cm, ps, s, c);
@@ -108,9 +124,15 @@ int choose(const CostModel& cm, const PairShape& ps, int s, int algo) {
             best_cls = c;
         }
     }
-    if (algo == NDC_BATCH_DIRECT) return -1;
-    if (algo == NDC_BATCH_FFT) return best_cls;   // -1 when no class holds the dilated kernel
-    return direct_cost(cm, ps, s) <= best ? -1 : best_cls;
+    if (algo == NDC_BATCH_DIRECT) return kMethodDirect;
+    if (algo == NDC_BATCH_FFT) return best_cls;   // direct when no class holds the dilated kernel
+    if (algo == NDC_BATCH_MFMA && mfma_eligible(ps, s)) return kMethodMfma;
+    int pick = direct_cost(cm, ps, s) <= best ? kMethodDirect : best_cls;
+    if (algo == NDC_BATCH_AUTO_BF16X3) {
+        const double vm = mfma_cost(cm, ps, s);
+        if (vm < std::min(best, direct_cost(cm, ps, s))) pick = kMethodMfma;
+    }
+    return pick;
 }
 
 struct Schedule {
@@ -150,13 +172,21 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
         bp.per = 0;
         S.main_bytes += 4.0 * double(ps.n + ps.out);
         if (ps.out <= 0) continue;
-        if (method[i] < 0) {
+        if (method[i] == kMethodDirect) {
             S.main_bytes += 4.0 * double(ps.k);
             S.main_flops += 2.0 * double(ps.out) * double(ps.k);
             for (int64_t o = 0; o < ps.out; o += tile) {
                 const int cnt = int(std::min<int64_t>(tile, ps.out - o));
                 BatchItem it{0, int32_t(i), int32_t(o), cnt};
                 work.emplace_back(cm.dir_a + cm.dir_b * double(cnt) * double(ps.k), it);
+            }
+        } else if (method[i] == kMethodMfma) {
+            S.main_bytes += 4.0 * double(ps.k);
+            S.main_flops += 2.0 * double(ps.out) * double(ps.k);
+            const double c = cm.mfma_a + cm.mfma_b * double(mfma_nkb(ps));
+            for (int64_t o = 0; o < ps.out; o += kBatchMfmaTile) {
+                const int cnt = int(std::min<int64_t>(kBatchMfmaTile, ps.out - o));
+                work.emplace_back(c, BatchItem{kBatchKindMfma, int32_t(i), int32_t(o), cnt});
             }
         } else {
             const int c = method[i];
@@ -324,6 +354,25 @@ bool calibrate(LaunchContext& ctx, int s, int d, CostModel* cm) {
         out.dir_b = std::max(1e-14, (nitems[0] * t[1] - nitems[1] * t[0]) / det);
         out.dir_a = std::max(0.0, (t[0] - work[0] * out.dir_b) / nitems[0]);
     }
+    // matrix-core items (stride 1 only): t = nitems (a + b nkb) at two kernel lengths
+    if (s == 1) {
+        const int64_t kcap = (kBatchMfmaMaxKdil - 1) / d + 1;
+        const int64_t mt[2] = {std::min<int64_t>(64, kcap), std::min<int64_t>(512, kcap)};
+        double tm[2] = {0, 0}, nm[2] = {0, 0}, kbm[2] = {0, 0};
+        for (int j = 0; j < 2; ++j) {
+            auto so = shapes(mt[j]);
+            std::vector<int> method(P, kMethodMfma);
+            Schedule S = build_schedule(so.first, so.second, method, s, out);
+            if (!time_schedule(ctx, S, x, k, y, s, d, &tm[j]) || S.items.empty()) return false;
+            nm[j] = double(S.items.size());
+            kbm[j] = double(mfma_nkb(so.first[0]));
+        }
+        const double pi0 = tm[0] / nm[0], pi1 = tm[1] / nm[1];   // ms per item
+        if (kbm[1] > kbm[0]) {
+            out.mfma_b = std::max(1e-12, (pi1 - pi0) / (kbm[1] - kbm[0]));
+            out.mfma_a = std::max(0.0, pi0 - out.mfma_b * kbm[0]);
+        }
+    }
     out.measured = true;
     *cm = out;
     return true;
@@ -331,7 +380,7 @@ bool calibrate(LaunchContext& ctx, int s, int d, CostModel* cm) {
 
 // The cost model of this handle for strides/dilation (s, d), measured once.
 CostModel cost_model(LaunchContext& ctx, int s, int d, bool measure) {
-    const std::string key = "batch_costs:" + std::to_string(s) + ":" + std::to_string(d);
+    const std::string key = "batch_costs2:" + std::to_string(s) + ":" + std::to_string(d);
     std::string& m = ctx.memo(key);
     CostModel cm = default_costs();
     if (m.size() == sizeof(CostModel)) {
@@ -416,7 +465,8 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
     if (d->dtype != NDC_F32) return fail(h, NDC_ERR_UNSUPPORTED, "batches are f32");
     if (d->count < 0 || (d->count > 0 && (!d->x_len || !d->k_len)))
         return fail(h, NDC_ERR_INVALID_ARGUMENT, "bad pair count or length arrays");
-    if (d->algo < NDC_BATCH_AUTO || d->algo > NDC_BATCH_PLAIN) return fail(h, NDC_ERR_INVALID_ARGUMENT, "unknown batch algorithm");
+    if (d->algo < NDC_BATCH_AUTO || d->algo > NDC_BATCH_AUTO_BF16X3)
+        return fail(h, NDC_ERR_INVALID_ARGUMENT, "unknown batch algorithm");
     if (d->flags & ~uint32_t(NDC_FLAG_HOST_MEMORY)) return fail(h, NDC_ERR_INVALID_ARGUMENT, "unknown batch flags");
     LaunchContext* ctx = handle_context(h);
     if (hipSetDevice(handle_device(h)) != hipSuccess) return fail(h, NDC_ERR_DEVICE, "hipSetDevice failed");
@@ -454,7 +504,7 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
     if (b->algo != NDC_BATCH_PLAIN) {
         if (int64_t(b->s) >= (int64_t(1) << 20) || int64_t(b->d) >= (int64_t(1) << 20))
             return fail(h, NDC_ERR_UNSUPPORTED, "stride or dilation too large for the batch kernels");
-        b->cm = cost_model(*ctx, b->s, b->d, b->algo == NDC_BATCH_AUTO);
+        b->cm = cost_model(*ctx, b->s, b->d, b->algo == NDC_BATCH_AUTO || b->algo == NDC_BATCH_AUTO_BF16X3);
         b->method.resize(size_t(P));
         for (int64_t i = 0; i < P; ++i) b->method[size_t(i)] = choose(b->cm, b->shapes[size_t(i)], b->s, b->algo);
         b->S = build_schedule(b->shapes, b->offs, b->method, b->s, b->cm);
@@ -560,14 +610,18 @@ std::string batch_report(const Batch& b) {
     for (int c = 0; c < kBatchClasses; ++c) append(r, c ? ", %.4e" : "%.4e", b.cm.fft_item[c]);
     r += "]";
     append(r, ", \"direct_item_ms\": %.4e", b.cm.dir_a);
-    append(r, ", \"direct_mac_ms\": %.4e}", b.cm.dir_b);
+    append(r, ", \"direct_mac_ms\": %.4e", b.cm.dir_b);
+    append(r, ", \"mfma_item_ms\": %.4e", b.cm.mfma_a);
+    append(r, ", \"mfma_kblock_ms\": %.4e}", b.cm.mfma_b);
     // choice per bucket: kernel-length octave x signal-length octave
     std::map<std::pair<int, int>, std::map<std::string, int64_t>> bk;
     for (size_t i = 0; i < b.shapes.size() && i < b.method.size(); ++i) {
         const int kb = int(std::floor(std::log2(double(std::max<int64_t>(1, b.shapes[i].k)))));
         const int nb = int(std::floor(std::log2(double(std::max<int64_t>(1, b.shapes[i].n)))));
         const int m = b.method[i];
-        const std::string name = m < 0 ? "direct" : "fft" + std::to_string(2 * batch_lc(m));
+        const std::string name = m == kMethodDirect ? "direct_fp32"
+                                 : m == kMethodMfma  ? "direct_bf16x3"
+                                                     : "fft" + std::to_string(2 * batch_lc(m));
         bk[{kb, nb}][name] += 1;
     }
     r += ", \"buckets\": [";

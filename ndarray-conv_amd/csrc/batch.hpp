@@ -33,6 +33,9 @@ constexpr int kBatchPoints = 4096;           // complex points per FFT item (G l
 constexpr int kBatchDirectTile = 2048;       // outputs per direct item (8 per thread)
 constexpr int kBatchWindow = 6144;           // LDS window capacity of a direct item (floats)
 constexpr int kBatchTapChunk = 1024;         // taps staged per round of a direct item
+constexpr int kBatchMfmaTile = 4096;         // outputs per matrix-core item (1024 per wave)
+constexpr int kBatchMfmaMaxKdil = 1792;      // longest dilated kernel of a matrix-core item (LDS)
+constexpr int kBatchKindMfma = 1 + kBatchClasses;
 
 constexpr int batch_lc(int cls) { return 256 << cls; }
 constexpr int batch_lines(int cls) { return kBatchPoints / batch_lc(cls); }
@@ -55,6 +58,7 @@ struct BatchPair {
 // One unit of the work queue (16 bytes).
 //   kind 0: direct tile  a = pair, b = first output, c = output count
 //   kind 1 + cls: FFT    a = first line, b = line count (<= batch_lines(cls))
+//   kind kBatchKindMfma: BF16x3 matrix-core tile, a = pair, b = first output, c = count
 struct BatchItem {
     int32_t kind, a, b, c;
 };

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "bf16_split.hpp"
 #include "border1d.hpp"
 #include "fft_bfly.hpp"
 
@@ -35,6 +36,12 @@ constexpr int kWinSlots = kBatchWindow + kBatchWindow / 8;       // padded direc
 constexpr size_t kLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
 static_assert(size_t(kWinSlots + kBatchTapChunk) * sizeof(float) <= size_t(kDataSlots) * sizeof(C),
               "direct-item staging must fit the FFT data region");
+// matrix-core item: padded window of 4096 + 16 nkb + 32 samples, taps kdil + 96
+constexpr int kMfmaNkbMax = (kBatchMfmaMaxKdil + 31 + 15) / 16;
+constexpr int kMfmaWin = kBatchMfmaTile + 16 * kMfmaNkbMax + 32;
+constexpr int kMfmaWinSlots = ((kMfmaWin + 4 * (kMfmaWin >> 5) + 3) / 4) * 4;
+static_assert(size_t(kMfmaWinSlots + kBatchMfmaMaxKdil + 96) * sizeof(uint32_t) <= size_t(kDataSlots) * sizeof(C),
+              "matrix-core staging must fit the FFT data region");
 
 // threadIdx.x behind an empty volatile asm: inside the persistent loop it is
 // re-read per item, so the compiler cannot hoist the address arithmetic of
@@ -47,6 +54,25 @@ __device__ __forceinline__ int tid() {
 
 __device__ __forceinline__ int padi(int i) { return i + (i >> 4); }
 __device__ __forceinline__ int padw(int i) { return i + (i >> 3); }
+
+// st(e, f(e)) for e < n, with U values of f (global loads) in flight per
+// thread before any is used: item stages are otherwise bound by load latency.
+template <int U, typename F, typename S>
+__device__ __forceinline__ void staged(int n, F&& f, S&& st) {
+    for (int base = tid(); base < n; base += U * T) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * T;
+            v[u] = e < n ? f(e) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * T;
+            if (e < n) st(e, v[u]);
+        }
+    }
+}
 
 // W_L^e from the quarter table q[r] = W_L^r, r < L/4 (L a power of two).
 __device__ __forceinline__ C twl(const C* q, int e, int L) {
@@ -155,7 +181,9 @@ __device__ __forceinline__ C c2r(C A, C B, C w) {
 // Per-line context of an item, staged in LDS.
 struct LineCtx {
     int64_t x_off, y_off, ks_off, k_off;
+    int64_t src;                   // x offset of window sample 0 when the window is all interior
     int32_t n, k, p0, pds, out, per, o0, valid;
+    int32_t interior, pad_;
 };
 
 __device__ __forceinline__ float pad_value(const BatchArgs& a, const LineCtx& c, int64_t P) {
@@ -175,7 +203,9 @@ __device__ __forceinline__ float corr_tap(const BatchArgs& a, int64_t k_off, int
     return a.k[k_off + (a.rev ? K - 1 - t : t)];
 }
 
-__device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, int G, int first, int count, bool kernel_lines) {
+// span: window length in samples (the interior test covers [o0 s, o0 s + span)).
+__device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, int G, int first, int count,
+                                              bool kernel_lines, int span) {
     const int g = tid();
     if (g < G) {
         LineCtx c{};
@@ -201,6 +231,9 @@ __device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, i
             c.pds = p.pds;
             c.out = p.out;
             c.per = p.per;
+            const int64_t P0 = int64_t(c.o0) * a.s;
+            c.interior = !kernel_lines && P0 >= c.p0 && P0 + span <= int64_t(c.p0) + c.n;
+            c.src = c.x_off + P0 - c.p0;
         }
         lc[g] = c;
     }
@@ -212,7 +245,7 @@ template <int LC, typename F>
 __device__ __forceinline__ void load_reals(float* fb, const LineCtx* lc, F&& val) {
     constexpr int G = kBatchPoints / LC;
     constexpr int L = 2 * LC;
-    constexpr int kU = 8;
+    constexpr int kU = 16;
     for (int base = tid(); base < G * L; base += kU * T) {
         float v[kU];
 #pragma unroll
@@ -236,15 +269,37 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
     constexpr int LS = LC + LC / 16;
     constexpr int L = 2 * LC;
     constexpr int H = LC / 2 + 1;
-    load_line_ctx(a, lc, G, it.a, it.b, false);
+    load_line_ctx(a, lc, G, it.a, it.b, false, L);
     __syncthreads();
     const int s = a.s;
     float* fb = reinterpret_cast<float*>(buf);
     // window of padded positions [o0 s, o0 s + L), border remap fused in
-    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) { return pad_value(a, c, int64_t(c.o0) * s + u); });
+    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) {
+        return c.interior ? a.x[c.src + u] : pad_value(a, c, int64_t(c.o0) * s + u);
+    });
+    // the kernel-spectrum bins this thread multiplies, loaded while the
+    // forward transform runs
+    constexpr int NI = (G * H + T - 1) / T;
+    C kmv[NI], kpv[NI];
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+        const int idx = tid() + r * T;
+        kmv[r] = kpv[r] = cmake<C>(0.f, 0.f);
+        if (idx < G * H) {
+            const int g = idx / H, m = idx - g * H;
+            const LineCtx& c = lc[g];
+            if (c.valid) {
+                kmv[r] = a.kspec[c.ks_off + m];
+                kpv[r] = a.kspec[c.ks_off + LC - m];
+            }
+        }
+    }
     fft_lds<LC>(buf, q, false);
     // spectrum product, bins m and LC - m together
-    for (int idx = tid(); idx < G * H; idx += T) {
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+        const int idx = tid() + r * T;
+        if (idx >= G * H) break;
         const int g = idx / H, m = idx - g * H;
         const LineCtx& c = lc[g];
         if (!c.valid) continue;
@@ -253,9 +308,8 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
         const C zm = Z[padi(m)];
         const C zp = Z[padi(mp == LC ? 0 : mp)];
         const C wm = twl(q, m, L), wp = twl(q, mp, L);
-        const C* K = a.kspec + c.ks_off;
-        C Pm = cmul(r2c(zm, zp, wm), K[m]);
-        C Pp = cmul(r2c(zp, zm, wp), K[mp]);
+        C Pm = cmul(r2c(zm, zp, wm), kmv[r]);
+        C Pp = cmul(r2c(zp, zm, wp), kpv[r]);
         if (m == 0) {   // imaginary parts of DC and Nyquist are ignored by c2r
             Pm.y = 0.f;
             Pp.y = 0.f;
@@ -281,7 +335,7 @@ __device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem&
     constexpr int G = kBatchPoints / LC;
     constexpr int LS = LC + LC / 16;
     constexpr int L = 2 * LC;
-    load_line_ctx(a, lc, G, it.first, it.count, true);
+    load_line_ctx(a, lc, G, it.first, it.count, true, 0);
     __syncthreads();
     const int d = a.d;
     float* fb = reinterpret_cast<float*>(buf);
@@ -329,8 +383,14 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
         const int nt = K - t0 < chunk ? K - t0 : chunk;
         const int64_t wbase = int64_t(o0) * s + int64_t(t0) * d;
         const int wlen = span + (nt - 1) * d;
-        for (int e = tid(); e < wlen; e += T) win[padw(e)] = pad_value(a, c, wbase + e);
-        for (int t = tid(); t < nt; t += T) taps[t] = corr_tap(a, p.k_off, K, t0 + t);
+        auto put = [&](int e, float v) { win[padw(e)] = v; };
+        if (wbase >= c.p0 && wbase + wlen <= int64_t(c.p0) + c.n) {
+            const float* src = a.x + c.x_off + (wbase - c.p0);   // all interior: no border logic
+            staged<12>(wlen, [&](int e) { return src[e]; }, put);
+        } else {
+            staged<12>(wlen, [&](int e) { return pad_value(a, c, wbase + e); }, put);
+        }
+        staged<4>(nt, [&](int t) { return corr_tap(a, p.k_off, K, t0 + t); }, [&](int t, float v) { taps[t] = v; });
         __syncthreads();
         if (fast) {
             // 8 consecutive outputs per thread, a sliding register window
@@ -363,6 +423,93 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
     for (int m = 0; m < 8; ++m) {
         const int o = fast ? 8 * tid() + m : tid() + m * T;
         if (o < cnt) y[o] = acc[m];
+    }
+}
+
+// BF16x3 matrix-core tile (stride 1): outputs [o0, o0 + cnt), cnt <= 4096,
+// 1024 per wave as one 32x32 tile D[n][m] = out[32 n + m] of the product
+//   D = A B,  A[n][j] = pad[32 n + j],  B[j][m] = kc[j - m]   (Toeplitz),
+// K = j running over kdil + 31 values in blocks of 16.  A rows come straight
+// from the staged window (eight consecutive samples per lane), B columns from
+// the staged taps; both are split hi/lo once when staged.  Each block issues
+// hi*hi + hi*lo + lo*hi with FP32 accumulation.
+__device__ __forceinline__ int padm(int e) { return e + 4 * (e >> 5); }
+
+__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& it, uint32_t* win, uint32_t* tp) {
+    const BatchPair p = a.pairs[it.a];
+    const int o0 = it.b, cnt = it.c;
+    const int d = a.d;
+    const int kdil = (p.k - 1) * d + 1;
+    const int nkb = (kdil + 31 + 15) / 16;
+    if (kdil > kBatchMfmaMaxKdil) return;   // the planner never queues these
+    LineCtx c{};
+    c.x_off = p.x_off;
+    c.n = p.n;
+    c.p0 = p.p0;
+    c.pds = p.pds;
+    // window: padded positions o0 + e for every sample an A operand reads,
+    // stored with 4 slots of padding per 32 (conflict-free 16-byte reads)
+    const int W = kBatchMfmaTile + 16 * nkb + 32;
+    auto put = [&](int e, float v) { win[padm(e)] = split_pack(v); };
+    if (int64_t(o0) >= c.p0 && int64_t(o0) + W <= int64_t(c.p0) + c.n) {
+        const float* src = a.x + c.x_off + (int64_t(o0) - c.p0);   // all interior: no border logic
+        staged<12>(W, [&](int e) { return src[e]; }, put);
+    } else {
+        staged<12>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
+    }
+    // taps: tp[32 + q] = kc_dil[q] for q in [0, kdil), zero around it
+    const int ntp = kdil + 96;
+    staged<4>(ntp, [&](int e) {
+        const int q = e - 32;
+        return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f;
+    }, [&](int e, float v) { tp[e] = split_pack(v); });
+    __syncthreads();
+    const int t = tid();
+    const int lane = t & 63, wave = t >> 6;
+    const int half = lane >> 5, m = lane & 31;
+    const int wb = wave * 1024;
+    // operands of K-block kb: A lane holds row m (a 32-output segment), B lane
+    // column m (an output offset); both K entries 16 kb + 8 half + i
+    auto load = [&](int kb, u32x4& ah, u32x4& al, u32x4& bh, u32x4& bl) {
+        const int ea = padm(wb + 32 * m + 16 * kb + 8 * half);
+        const u32x4 a0 = *reinterpret_cast<const u32x4*>(win + ea);
+        const u32x4 a1 = *reinterpret_cast<const u32x4*>(win + ea + 4);
+        const uint32_t pa[8] = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+        const int q0 = 32 + 16 * kb + 8 * half - m;
+        uint32_t pb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) pb[i] = tp[q0 + i];
+        unpack8(pa, ah, al);
+        unpack8(pb, bh, bl);
+    };
+    // three independent accumulation chains (hi*hi, hi*lo, lo*hi), summed at
+    // the end, so consecutive products do not wait on each other
+    floatx16 acc_hh, acc_hl, acc_lh;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc_hh[i] = acc_hl[i] = acc_lh[i] = 0.f;
+    u32x4 ah, al, bh, bl;
+    load(0, ah, al, bh, bl);
+    for (int kb = 0; kb < nkb; ++kb) {
+        u32x4 nah = ah, nal = al, nbh = bh, nbl = bl;
+        if (kb + 1 < nkb) load(kb + 1, nah, nal, nbh, nbl);   // next block's operands in flight
+        acc_hh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah), __builtin_bit_cast(bf16x8, bh),
+                                                        acc_hh, 0, 0, 0);
+        acc_hl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah), __builtin_bit_cast(bf16x8, bl),
+                                                        acc_hl, 0, 0, 0);
+        acc_lh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al), __builtin_bit_cast(bf16x8, bh),
+                                                        acc_lh, 0, 0, 0);
+        ah = nah;
+        al = nal;
+        bh = nbh;
+        bl = nbl;
+    }
+    // D lane holds D[8 (i/4) + 4 half + i%4][m]
+    float* y = a.y + p.y_off + o0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int o = wb + 32 * (8 * (i / 4) + 4 * half + (i % 4)) + m;
+        const float v = acc_hh[i] + (acc_hl[i] + acc_lh[i]);
+        if (o < cnt) y[o] = v;
     }
 }
 
@@ -409,6 +556,9 @@ __global__ void __launch_bounds__(kBatchThreads) batch_main_kernel(BatchArgs a) 
         const BatchItem it = a.items[idx];
         if (it.kind == 0) {
             direct_item(a, it, win, taps);
+        } else if (it.kind == kBatchKindMfma) {
+            uint32_t* mwin = reinterpret_cast<uint32_t*>(buf);
+            mfma_item(a, it, mwin, mwin + kMfmaWinSlots);
         } else {
             const int cls = it.kind - 1;
             if (cls != cur) {

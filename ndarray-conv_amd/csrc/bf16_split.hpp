@@ -1,0 +1,39 @@
+// BF16 operands for the matrix cores: round-to-nearest-even conversion and
+// the hi/lo split of the "BF16x3" products (a*b ~ ah*bh + ah*bl + al*bh).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ndc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Round to nearest even bf16 (the bit pattern in the low 16 bits).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// f = hi + lo (+ a residual of at most 2^-17 |f|), packed as hi | lo << 16.
+__device__ __forceinline__ uint32_t split_pack(float f) {
+    const uint32_t h = bf16_bits(f);
+    const float hf = __builtin_bit_cast(float, h << 16);
+    const uint32_t l = bf16_bits(f - hf);
+    return h | (l << 16);
+}
+
+// Eight packed (hi | lo << 16) values to the two 8-element bf16 operands:
+// hi[i] holds elements 2i, 2i+1 of the hi parts, lo[i] of the lo parts.
+__device__ __forceinline__ void unpack8(const uint32_t* p, u32x4& hi, u32x4& lo) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        hi[i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+        lo[i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+    }
+}
+
+}  // namespace ndc

@@ -22,15 +22,12 @@
 #include <cmath>
 #include <cstring>
 
+#include "bf16_split.hpp"
 #include "device_common.hpp"
 #include "launch.hpp"
 
 namespace ndc {
 namespace {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
@@ -64,20 +61,6 @@ struct MfmaArgs {
     int32_t npass_c;         // column passes
     int32_t npass_r;         // row groups of 32
 };
-
-// Round to nearest even bf16 (the bit pattern in the low 16 bits).
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    const uint32_t u = __builtin_bit_cast(uint32_t, f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-
-// f = hi + lo (+ a residual of at most 2^-17 |f|), packed as hi | lo << 16.
-__device__ __forceinline__ uint32_t split_pack(float f) {
-    const uint32_t h = bf16_bits(f);
-    const float hf = __builtin_bit_cast(float, h << 16);
-    const uint32_t l = bf16_bits(f - hf);
-    return h | (l << 16);
-}
 
 // Value of the padded input from the border codes of its row and column
 // (codes resolved once per pass into LDS).

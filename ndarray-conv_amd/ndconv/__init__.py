@@ -490,7 +490,8 @@ def conv_fft_par(data, kernel, conv_mode, padding_mode):
 # reference; one persistent launch here)
 # ---------------------------------------------------------------------------
 
-_BATCH_ALGOS = {"auto": C.BATCH_AUTO, "fft": C.BATCH_FFT, "direct": C.BATCH_DIRECT, "plain": C.BATCH_PLAIN}
+_BATCH_ALGOS = {"auto": C.BATCH_AUTO, "fft": C.BATCH_FFT, "direct": C.BATCH_DIRECT, "plain": C.BATCH_PLAIN,
+                "bf16x3": C.BATCH_MFMA, "auto_bf16x3": C.BATCH_AUTO_BF16X3}
 
 
 def _batch_desc(x_lens, k_lens, conv_mode, padding_mode, dilation, reverse_flag, algo, host):
@@ -521,8 +522,10 @@ class BatchPlan:
     Each pair has the semantics of one conv_fft call (conv_fft/mod.rs:113-118)
     with the shared ConvMode, PaddingMode, dilation and reverse flag.  Signals,
     kernels and outputs are packed back to back in pair order.  algo: "auto"
-    (device-measured cost model picks direct or overlap-save FFT per pair),
-    "fft", "direct", or "plain" (one single-problem conv_fft per pair).
+    (a device-measured cost model picks FP32 direct or FP32 overlap-save FFT per
+    pair), "auto_bf16x3" (the same choice with BF16x3 matrix-core direct as a
+    third form), "fft", "direct" (FP32), "bf16x3", or "plain" (one
+    single-problem conv_fft per pair).
     host=True takes numpy arrays; otherwise torch GPU tensors, ordered on
     torch's current stream."""
 

@@ -25,7 +25,7 @@ FLAG_HOST_MEMORY, FLAG_X_STRIDED, FLAG_K_STRIDED = 1, 2, 4
 # ndc_op
 OP_CONV, OP_CONV_FFT = 0, 1
 # ndc_batch_algo
-BATCH_AUTO, BATCH_FFT, BATCH_DIRECT, BATCH_PLAIN = range(4)
+BATCH_AUTO, BATCH_FFT, BATCH_DIRECT, BATCH_PLAIN, BATCH_MFMA, BATCH_AUTO_BF16X3 = range(6)
 
 I64A = ctypes.c_int64 * MAX_RANK
 
