@@ -34,7 +34,7 @@ constexpr int kBatchDirectTile = 2048;       // outputs per direct item (8 per t
 constexpr int kBatchWindow = 6144;           // LDS window capacity of a direct item (floats)
 constexpr int kBatchTapChunk = 1024;         // taps staged per round of a direct item
 constexpr int kBatchMfmaTile = 4096;         // outputs per matrix-core item (1024 per wave)
-constexpr int kBatchMfmaMaxKdil = 1792;      // longest dilated kernel of a matrix-core item (LDS)
+constexpr int kBatchMfmaMaxKdil = 1024;      // longest dilated kernel of a matrix-core item (LDS)
 constexpr int kBatchKindMfma = 1 + kBatchClasses;
 
 constexpr int batch_lc(int cls) { return 256 << cls; }

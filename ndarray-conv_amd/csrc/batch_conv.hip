@@ -36,12 +36,6 @@ constexpr int kWinSlots = kBatchWindow + kBatchWindow / 8;       // padded direc
 constexpr size_t kLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
 static_assert(size_t(kWinSlots + kBatchTapChunk) * sizeof(float) <= size_t(kDataSlots) * sizeof(C),
               "direct-item staging must fit the FFT data region");
-// matrix-core item: padded window of 4096 + 16 nkb + 32 samples, taps kdil + 96
-constexpr int kMfmaNkbMax = (kBatchMfmaMaxKdil + 31 + 15) / 16;
-constexpr int kMfmaWin = kBatchMfmaTile + 16 * kMfmaNkbMax + 32;
-constexpr int kMfmaWinSlots = ((kMfmaWin + 4 * (kMfmaWin >> 5) + 3) / 4) * 4;
-static_assert(size_t(kMfmaWinSlots + kBatchMfmaMaxKdil + 96) * sizeof(uint32_t) <= size_t(kDataSlots) * sizeof(C),
-              "matrix-core staging must fit the FFT data region");
 
 // threadIdx.x behind an empty volatile asm: inside the persistent loop it is
 // re-read per item, so the compiler cannot hoist the address arithmetic of
@@ -245,7 +239,7 @@ template <int LC, typename F>
 __device__ __forceinline__ void load_reals(float* fb, const LineCtx* lc, F&& val) {
     constexpr int G = kBatchPoints / LC;
     constexpr int L = 2 * LC;
-    constexpr int kU = 16;
+    constexpr int kU = 32;   // every sample of the item in one round
     for (int base = tid(); base < G * L; base += kU * T) {
         float v[kU];
 #pragma unroll
@@ -386,9 +380,9 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
         auto put = [&](int e, float v) { win[padw(e)] = v; };
         if (wbase >= c.p0 && wbase + wlen <= int64_t(c.p0) + c.n) {
             const float* src = a.x + c.x_off + (wbase - c.p0);   // all interior: no border logic
-            staged<12>(wlen, [&](int e) { return src[e]; }, put);
+            staged<24>(wlen, [&](int e) { return src[e]; }, put);
         } else {
-            staged<12>(wlen, [&](int e) { return pad_value(a, c, wbase + e); }, put);
+            staged<24>(wlen, [&](int e) { return pad_value(a, c, wbase + e); }, put);
         }
         staged<4>(nt, [&](int t) { return corr_tap(a, p.k_off, K, t0 + t); }, [&](int t, float v) { taps[t] = v; });
         __syncthreads();
@@ -429,40 +423,70 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
 // BF16x3 matrix-core tile (stride 1): outputs [o0, o0 + cnt), cnt <= 4096,
 // 1024 per wave as one 32x32 tile D[n][m] = out[32 n + m] of the product
 //   D = A B,  A[n][j] = pad[32 n + j],  B[j][m] = kc[j - m]   (Toeplitz),
-// K = j running over kdil + 31 values in blocks of 16.  A rows come straight
-// from the staged window (eight consecutive samples per lane), B columns from
-// the staged taps; both are split hi/lo once when staged.  Each block issues
-// hi*hi + hi*lo + lo*hi with FP32 accumulation.
-__device__ __forceinline__ int padm(int e) { return e + 4 * (e >> 5); }
+// K = j running over kdil + 31 values in blocks of 16.  The window and the
+// taps are split hi/lo once, when staged, into separate bf16 planes, so an A
+// operand is one aligned 16-byte read per plane; the taps are kept in two
+// copies shifted by one element so every B read is 4-byte aligned.  Each
+// block issues hi*hi, hi*lo and lo*hi into three independent accumulators.
+__device__ __forceinline__ int padh(int e) { return e + 8 * (e >> 5); }   // 16 bytes of padding per 64
 
-__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& it, uint32_t* win, uint32_t* tp) {
+struct MfmaLds {
+    static constexpr int nkb = (kBatchMfmaMaxKdil + 31 + 15) / 16;
+    static constexpr int win = kBatchMfmaTile + 16 * nkb + 32;         // window samples
+    static constexpr int win_slots = win + 8 * ((win + 31) / 32);      // padded plane length
+    static constexpr int taps = 16 * nkb + 48;                         // tap plane length (even)
+    static constexpr size_t bytes = size_t(2 * win_slots + 4 * taps) * 2;
+};
+
+__device__ __forceinline__ uint16_t bf16_hi(uint32_t packed) { return uint16_t(packed & 0xFFFFu); }
+__device__ __forceinline__ uint16_t bf16_lo(uint32_t packed) { return uint16_t(packed >> 16); }
+
+__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& it, uint16_t* lds) {
     const BatchPair p = a.pairs[it.a];
     const int o0 = it.b, cnt = it.c;
     const int d = a.d;
     const int kdil = (p.k - 1) * d + 1;
     const int nkb = (kdil + 31 + 15) / 16;
     if (kdil > kBatchMfmaMaxKdil) return;   // the planner never queues these
+    uint16_t* wh = lds;                                   // window, hi plane
+    uint16_t* wl = wh + MfmaLds::win_slots;               // window, lo plane
+    uint16_t* th0 = wl + MfmaLds::win_slots;              // taps hi, tap q at 32 + q
+    uint16_t* th1 = th0 + MfmaLds::taps;                  // taps hi, shifted: tap q at 31 + q
+    uint16_t* tl0 = th1 + MfmaLds::taps;
+    uint16_t* tl1 = tl0 + MfmaLds::taps;
     LineCtx c{};
     c.x_off = p.x_off;
     c.n = p.n;
     c.p0 = p.p0;
     c.pds = p.pds;
-    // window: padded positions o0 + e for every sample an A operand reads,
-    // stored with 4 slots of padding per 32 (conflict-free 16-byte reads)
+    // window: padded positions o0 + e for every sample an A operand reads
     const int W = kBatchMfmaTile + 16 * nkb + 32;
-    auto put = [&](int e, float v) { win[padm(e)] = split_pack(v); };
+    auto put = [&](int e, float v) {
+        const uint32_t s2 = split_pack(v);
+        wh[padh(e)] = bf16_hi(s2);
+        wl[padh(e)] = bf16_lo(s2);
+    };
     if (int64_t(o0) >= c.p0 && int64_t(o0) + W <= int64_t(c.p0) + c.n) {
         const float* src = a.x + c.x_off + (int64_t(o0) - c.p0);   // all interior: no border logic
-        staged<12>(W, [&](int e) { return src[e]; }, put);
+        staged<24>(W, [&](int e) { return src[e]; }, put);
     } else {
-        staged<12>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
+        staged<24>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
     }
-    // taps: tp[32 + q] = kc_dil[q] for q in [0, kdil), zero around it
-    const int ntp = kdil + 96;
-    staged<4>(ntp, [&](int e) {
-        const int q = e - 32;
-        return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f;
-    }, [&](int e, float v) { tp[e] = split_pack(v); });
+    // taps: plane0[e] = kc_dil[e - 32], plane1[e] = kc_dil[e - 31], zero outside [0, kdil)
+    const int ntp = 16 * nkb + 48;
+    auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
+    // one load per tap: plane0[e] and plane1[e - 1] hold the same tap
+    staged<5>(ntp + 1, [&](int e) { return tapv(e - 32); }, [&](int e, float v) {
+        const uint32_t s2 = split_pack(v);
+        if (e < ntp) {
+            th0[e] = bf16_hi(s2);
+            tl0[e] = bf16_lo(s2);
+        }
+        if (e >= 1) {
+            th1[e - 1] = bf16_hi(s2);
+            tl1[e - 1] = bf16_lo(s2);
+        }
+    });
     __syncthreads();
     const int t = tid();
     const int lane = t & 63, wave = t >> 6;
@@ -470,20 +494,23 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     const int wb = wave * 1024;
     // operands of K-block kb: A lane holds row m (a 32-output segment), B lane
     // column m (an output offset); both K entries 16 kb + 8 half + i
+    const int par = (32 + 8 * half - m) & 1;   // parity of this lane's first tap index
+    const uint16_t* tbh = par ? th1 : th0;
+    const uint16_t* tbl = par ? tl1 : tl0;
     auto load = [&](int kb, u32x4& ah, u32x4& al, u32x4& bh, u32x4& bl) {
-        const int ea = padm(wb + 32 * m + 16 * kb + 8 * half);
-        const u32x4 a0 = *reinterpret_cast<const u32x4*>(win + ea);
-        const u32x4 a1 = *reinterpret_cast<const u32x4*>(win + ea + 4);
-        const uint32_t pa[8] = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-        const int q0 = 32 + 16 * kb + 8 * half - m;
-        uint32_t pb[8];
+        const int ea = padh(wb + 32 * m + 16 * kb + 8 * half);
+        ah = *reinterpret_cast<const u32x4*>(wh + ea);
+        al = *reinterpret_cast<const u32x4*>(wl + ea);
+        // taps 16 kb + 8 half - m + i, i < 8, from the copy where they start even
+        const int q = 32 + 16 * kb + 8 * half - m - par;
+        const uint32_t* ph = reinterpret_cast<const uint32_t*>(tbh + q);
+        const uint32_t* pl = reinterpret_cast<const uint32_t*>(tbl + q);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) pb[i] = tp[q0 + i];
-        unpack8(pa, ah, al);
-        unpack8(pb, bh, bl);
+        for (int i = 0; i < 4; ++i) {
+            bh[i] = ph[i];
+            bl[i] = pl[i];
+        }
     };
-    // three independent accumulation chains (hi*hi, hi*lo, lo*hi), summed at
-    // the end, so consecutive products do not wait on each other
     floatx16 acc_hh, acc_hl, acc_lh;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc_hh[i] = acc_hl[i] = acc_lh[i] = 0.f;
@@ -513,6 +540,8 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     }
 }
 
+static_assert(MfmaLds::bytes <= size_t(kDataSlots) * sizeof(C), "matrix-core staging must fit the FFT data region");
+
 __device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
     for (int i = tid(); i < n; i += T) dst[i] = src[i];
 }
@@ -539,26 +568,31 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a)
 
 // Persistent: every workgroup takes items from the queue head until the
 // queue is empty (the only exit, reached by every workgroup).
-__global__ void __launch_bounds__(kBatchThreads) batch_main_kernel(BatchArgs a) {
+__global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs a) {
     extern __shared__ C smem[];
     __shared__ LineCtx lc[16];
-    __shared__ int s_item;
+    __shared__ int s_item[2];
     C* q = smem;
     C* buf = smem + kTwSlots;
     float* win = reinterpret_cast<float*>(buf);
     float* taps = win + kWinSlots;
     int cur = -1;   // class whose twiddle table is in LDS
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int idx = s_item;
-        if (idx >= a.nitems) break;
+    // The queue head is popped one item ahead: the pop for the next item is
+    // in flight while the current one is processed.  s_item alternates
+    // between two slots so the write for the next item never overlaps the
+    // reads of the current index.
+    int next = 0;
+    if (threadIdx.x == 0) s_item[0] = atomicAdd(a.counter, 1);
+    __syncthreads();
+    for (int iter = 0;; ++iter) {
+        const int idx = s_item[iter & 1];
+        if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
+        if (threadIdx.x == 0) next = atomicAdd(a.counter, 1);
         const BatchItem it = a.items[idx];
         if (it.kind == 0) {
             direct_item(a, it, win, taps);
         } else if (it.kind == kBatchKindMfma) {
-            uint32_t* mwin = reinterpret_cast<uint32_t*>(buf);
-            mfma_item(a, it, mwin, mwin + kMfmaWinSlots);
+            mfma_item(a, it, reinterpret_cast<uint16_t*>(buf));
         } else {
             const int cls = it.kind - 1;
             if (cls != cur) {
@@ -574,6 +608,7 @@ __global__ void __launch_bounds__(kBatchThreads) batch_main_kernel(BatchArgs a) 
                 default: fft_item<4096>(a, it, q, buf, lc); break;
             }
         }
+        if (threadIdx.x == 0) s_item[(iter + 1) & 1] = next;
         __syncthreads();
     }
 }
