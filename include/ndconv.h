@@ -195,6 +195,26 @@ ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, cons
  * threading and planner reuse, which the handle subsumes. */
 ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
 
+/* The FFT processor's plain transforms (Processor::forward / backward,
+ * processor/mod.rs:83-119; real.rs:105-155,233-281; complex.rs:33-145) on a
+ * C-contiguous array.  The handle is the processor.
+ *  - dtype NDC_F32/NDC_F64: real input (forward) or real output (backward);
+ *    NDC_C64/NDC_C128: complex both ways.
+ *  - forward: x has `shape` [F0, ..., F(N-1)]; y receives the N-d DFT
+ *    (unnormalised, e^{-2 pi i}) with its axes rotated left by one:
+ *    shape [F1, ..., F(N-1)', F0], where F(N-1)' = F(N-1)/2 + 1 for real input.
+ *    A real forward records F(N-1) in the handle (the reference's rp_origin_len).
+ *  - backward: `spec_shape` is such a spectrum's shape; y receives the inverse,
+ *    divided by the number of output elements, in natural order; a real
+ *    backward restores the last axis to the recorded length (the imaginary parts
+ *    of the DC and Nyquist bins are ignored, as realfft's result is used there).
+ * Lines longer than one workgroup's LDS holds return NDC_ERR_UNSUPPORTED. */
+ndc_status ndc_fft_forward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* shape, const void* x, void* y,
+                           uint32_t flags);
+ndc_status ndc_fft_backward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* spec_shape, const void* spec,
+                            void* y, uint32_t flags);
+ndc_status ndc_fft_origin_len(ndc_handle h, int64_t* len);
+
 /* Per-kernel profile (no reference counterpart: the reference has no
  * instrumentation).  While enabled, every kernel launch on the handle is
  * bracketed by HIP events on the handle's stream.  The report synchronizes

@@ -36,6 +36,7 @@ public:
     std::unordered_map<std::string, std::pair<DevBuf, std::vector<int64_t>>> maps;
     std::unordered_map<std::string, Plan> plans;   // keyed by the raw descriptor bytes and op
     std::string last;
+    int64_t rp_origin_len = 0;   // real length of the last axis of the last real forward transform
 
     // per-kernel profile: HIP events recorded on the handle's stream around
     // every launch while enabled
@@ -252,6 +253,64 @@ ndc_status run(ndc_handle hh, const ndc_conv_desc* d, int op, const void* x, con
     return NDC_OK;
 }
 
+// The processor's forward/backward (processor/mod.rs:83-119).
+ndc_status run_processor(ndc_handle hh, bool forward, int32_t dtype, int32_t rank, const int64_t* shape,
+                         const void* x, void* y, uint32_t flags) {
+    if (!hh) return NDC_ERR_INVALID_ARGUMENT;
+    Handle& h = *reinterpret_cast<Handle*>(hh);
+    if (dtype != NDC_F32 && dtype != NDC_F64 && dtype != NDC_C64 && dtype != NDC_C128)
+        return set_err(hh, NDC_ERR_UNSUPPORTED, "the processor transforms f32/f64 and complex64/complex128");
+    if (rank < 1 || rank > kMaxRank || !shape) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "bad rank or shape");
+    if (flags & ~uint32_t(NDC_FLAG_HOST_MEMORY)) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "unknown flags");
+    int64_t n_in = 1;
+    for (int i = 0; i < rank; ++i) {
+        if (shape[i] < 1 || shape[i] > (int64_t(1) << 30))
+            return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "every axis needs 1 .. 2^30 elements");
+        n_in *= shape[i];
+    }
+    const bool real = dtype == NDC_F32 || dtype == NDC_F64;
+    const size_t rsz = (dtype == NDC_F64 || dtype == NDC_C128) ? 8 : 4;
+    const size_t csz = 2 * rsz;
+    size_t in_bytes, out_bytes;
+    int64_t origin = h.rp_origin_len;
+    if (forward) {
+        const int64_t H = real ? shape[rank - 1] / 2 + 1 : shape[rank - 1];
+        in_bytes = size_t(n_in) * (real ? rsz : csz);
+        out_bytes = size_t(n_in / shape[rank - 1] * H) * csz;
+    } else {
+        if (real && origin <= 0)
+            return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "backward needs a real forward transform on this processor first");
+        in_bytes = size_t(n_in) * csz;
+        // natural last axis: the spectrum's second-to-last axis (its first for rank 1)
+        const int64_t hl = rank == 1 ? shape[0] : shape[rank - 2];
+        out_bytes = real ? size_t(n_in / hl * origin) * rsz : size_t(n_in) * csz;
+    }
+    if (!x || !y) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "null data pointer");
+    if (hipSetDevice(h.device) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "hipSetDevice failed");
+    const bool host = (flags & NDC_FLAG_HOST_MEMORY) != 0;
+    const void* dx = x;
+    void* dy = y;
+    if (host) {
+        void* bx = h.grow(h.io[0], in_bytes);
+        void* by = h.grow(h.io[2], out_bytes);
+        if (!bx || !by) return set_err(hh, NDC_ERR_OUT_OF_MEMORY, "device buffer allocation failed");
+        if (hipMemcpyAsync(bx, x, in_bytes, hipMemcpyHostToDevice, h.stream()) != hipSuccess)
+            return set_err(hh, NDC_ERR_DEVICE, "host to device copy failed");
+        dx = bx;
+        dy = by;
+    }
+    h.err.clear();
+    const ndc_status s = launch_fft_processor(dtype, forward, rank, shape, origin, dx, dy, h);
+    if (s != NDC_OK) return set_err(hh, s, h.err);
+    if (forward && real) h.rp_origin_len = shape[rank - 1];
+    if (host) {
+        if (hipMemcpyAsync(y, dy, out_bytes, hipMemcpyDeviceToHost, h.stream()) != hipSuccess ||
+            hipStreamSynchronize(h.stream()) != hipSuccess)
+            return set_err(hh, NDC_ERR_DEVICE, "device to host copy failed");
+    }
+    return NDC_OK;
+}
+
 }  // namespace
 
 LaunchContext* handle_context(ndc_handle h) { return reinterpret_cast<Handle*>(h); }
@@ -390,6 +449,22 @@ ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, cons
 
 ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y) {
     return ndc::run(h, desc, NDC_OP_CONV_FFT, x, k, y);
+}
+
+ndc_status ndc_fft_forward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* shape, const void* x, void* y,
+                           uint32_t flags) {
+    return ndc::run_processor(h, true, dtype, rank, shape, x, y, flags);
+}
+
+ndc_status ndc_fft_backward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* spec_shape, const void* spec,
+                            void* y, uint32_t flags) {
+    return ndc::run_processor(h, false, dtype, rank, spec_shape, spec, y, flags);
+}
+
+ndc_status ndc_fft_origin_len(ndc_handle h, int64_t* len) {
+    if (!h || !len) return NDC_ERR_INVALID_ARGUMENT;
+    *len = reinterpret_cast<Handle*>(h)->rp_origin_len;
+    return NDC_OK;
 }
 
 ndc_status ndc_profile_enable(ndc_handle hh, int32_t on) {

@@ -1450,6 +1450,304 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     return NDC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Plain N-d transforms of the public processor (processor/real.rs,
+// processor/complex.rs): line transforms along one axis of a contiguous
+// array viewed as [outer][n][inner], and a 2-D transpose for the axis
+// rotation of the reference's spectral layout.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+struct LineFftArgs {
+    const Cx<R>* in;     // complex input (c2c, c2r)
+    const R* rin;        // real input (r2c)
+    Cx<R>* out;          // complex output (c2c, r2c)
+    R* rout;             // real output (c2r)
+    int64_t outer, inner;
+    int32_t n;           // elements along the axis: complex length (c2c) or real length F (r2c, c2r)
+    int32_t H;           // F/2 + 1 (r2c, c2r)
+    int32_t mode;        // 0 c2c, 1 r2c, 2 c2r
+    int32_t Ct;          // lines per block: along inner (columns) or consecutive rows (inner == 1)
+    int32_t inv;
+    int32_t ntiles;      // column tiles per outer index
+    R scale;
+    LinePlan<R> pl;      // complex length transformed: F/2 for even real lines, else n
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a) {
+    using C = Cx<R>;
+    extern __shared__ unsigned char smem_raw[];
+    const int L = a.pl.L;
+    const int Ct = a.Ct;
+    C* tw = reinterpret_cast<C*>(smem_raw);
+    C* b0 = tw + a.pl.ntw;
+    C* b1 = b0 + L * Ct;
+    const int T = blockDim.x;
+    const bool inv = a.inv != 0;
+    stage_table(tw, a.pl.tw, a.pl.ntw);
+    if (a.inner > 1) {
+        // columns: Ct lines at consecutive inner indices, element (c, j) at j*Ct + c
+        const int64_t o = blockIdx.x / a.ntiles;
+        const int64_t i0 = int64_t(blockIdx.x % a.ntiles) * Ct;
+        const int nc = int(imin64(Ct, a.inner - i0));
+        const int64_t base = o * int64_t(a.n) * a.inner + i0;
+        const float rC = 1.0f / float(Ct);
+        for (int b = threadIdx.x; b < L * Ct; b += kU * T) {
+            C v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int it = b + u * T;
+                v[u] = cmake<C>(R(0), R(0));
+                if (it < L * Ct) {
+                    const int j = fdiv(it, Ct, rC), c = it - j * Ct;
+                    if (c < nc) v[u] = a.in[base + int64_t(j) * a.inner + c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int it = b + u * T;
+                if (it < L * Ct) b0[it] = v[u];
+            }
+        }
+        __syncthreads();
+        const C* res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, inv) ? b1 : b0;
+        for (int it = threadIdx.x; it < L * Ct; it += T) {
+            const int j = fdiv(it, Ct, rC), c = it - j * Ct;
+            if (c < nc) a.out[base + int64_t(j) * a.inner + c] = cscale(res[it], a.scale);
+        }
+        return;
+    }
+    // rows: Ct consecutive rows, element (b, j) at b*L + j
+    const int64_t r0 = int64_t(blockIdx.x) * Ct;
+    const int nr = int(imin64(Ct, a.outer - r0));
+    const bool half = a.mode != 0 && L * 2 == a.n;   // even real line: half-length transform
+    const int tot = L * Ct;
+    const float rL = 1.0f / float(L);
+    for (int b = threadIdx.x; b < tot; b += kU * T) {
+        C v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = b + u * T;
+            v[u] = cmake<C>(R(0), R(0));
+            if (it < tot) {
+                const int rb = fdiv(it, L, rL), j = it - rb * L;
+                if (rb < nr) {
+                    const int64_t row = r0 + rb;
+                    if (a.mode == 0) {
+                        v[u] = a.in[row * a.n + j];
+                    } else if (a.mode == 1) {
+                        const R* x = a.rin + row * a.n;
+                        v[u] = half ? cmake<C>(x[2 * j], x[2 * j + 1]) : cmake<C>(x[j], R(0));
+                    } else {
+                        // c2r: bins of the full (odd) or half-length (even) inverse
+                        const C* X = a.in + row * a.H;
+                        if (half) {
+                            const int M = L;
+                            C A = X[j], B = cconj(X[M - j]);
+                            if (j == 0) {   // imaginary parts of DC and Nyquist are ignored
+                                A.y = R(0);
+                                B.y = R(0);
+                            }
+                            v[u] = c2r_pre(A, B, j, tw + a.pl.post);
+                        } else {
+                            C Y = j < a.H ? X[j] : cconj(X[a.n - j]);
+                            if (j == 0) Y.y = R(0);
+                            v[u] = Y;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = b + u * T;
+            if (it < tot) b0[it] = v[u];
+        }
+    }
+    __syncthreads();
+    const C* res = fft_lines<R, false>(b0, b1, a.pl, tw, Ct, L, 1, inv) ? b1 : b0;
+    if (a.mode == 0) {
+        for (int it = threadIdx.x; it < tot; it += T) {
+            const int rb = fdiv(it, L, rL), j = it - rb * L;
+            if (rb < nr) a.out[(r0 + rb) * a.n + j] = cscale(res[it], a.scale);
+        }
+    } else if (a.mode == 1) {
+        const int Hb = a.H;
+        for (int it = threadIdx.x; it < Ct * Hb; it += T) {
+            const int rb = it / Hb, k = it - rb * Hb;
+            if (rb >= nr) continue;
+            const C* Z = res + rb * L;
+            const C X = half ? r2c_bin(Z, L, k, tw + a.pl.post) : Z[k];
+            a.out[(r0 + rb) * Hb + k] = cscale(X, a.scale);
+        }
+    } else {
+        for (int it = threadIdx.x; it < tot; it += T) {
+            const int rb = fdiv(it, L, rL), j = it - rb * L;
+            if (rb >= nr) continue;
+            R* y = a.rout + (r0 + rb) * a.n;
+            if (half) {
+                y[2 * j] = res[it].x * a.scale;
+                y[2 * j + 1] = res[it].y * a.scale;
+            } else {
+                y[j] = res[it].x * a.scale;
+            }
+        }
+    }
+}
+
+// out[B][A] = in[A][B] (complex), 32x32 tiles through LDS.
+template <typename R>
+__global__ void __launch_bounds__(256) transpose_kernel(const Cx<R>* in, Cx<R>* out, int64_t A, int64_t B) {
+    __shared__ Cx<R> tile[32][33];
+    const int64_t a0 = int64_t(blockIdx.y) * 32, b0 = int64_t(blockIdx.x) * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int64_t ai = a0 + r, bi = b0 + threadIdx.x;
+        if (ai < A && bi < B) tile[r][threadIdx.x] = in[ai * B + bi];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int64_t bi = b0 + r, ai = a0 + threadIdx.x;
+        if (ai < A && bi < B) out[bi * A + ai] = tile[threadIdx.x][r];
+    }
+}
+
+// One line-transform launch along an axis of [outer][n][inner].
+template <typename R>
+ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
+                    int64_t inner, R scale) {
+    using C = Cx<R>;
+    LineFftArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    const bool half = mode != 0 && n % 2 == 0;
+    const int L = half ? n / 2 : n;
+    if (!make_lineplan(&a.pl, L, half ? n : 0, ctx)) {
+        ctx.err = "FFT plan failed";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    a.in = static_cast<const C*>(in);
+    a.rin = static_cast<const R*>(in);
+    a.out = static_cast<C*>(out);
+    a.rout = static_cast<R*>(out);
+    a.outer = outer;
+    a.inner = inner;
+    a.n = n;
+    a.H = n / 2 + 1;
+    a.mode = mode;
+    a.inv = inv ? 1 : 0;
+    a.scale = scale;
+    const size_t es = sizeof(C);
+    const size_t lds = ctx.max_lds();
+    int Ct = 1;
+    int64_t blocks = 0;
+    if (inner > 1) {
+        Ct = int(std::min<int64_t>(16, inner));
+        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es > lds) Ct /= 2;
+        a.ntiles = int32_t((inner + Ct - 1) / Ct);
+        blocks = outer * a.ntiles;
+    } else {
+        Ct = int(std::max<int64_t>(1, std::min<int64_t>(kMaxLpb, 512 / std::max(1, L))));
+        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es > lds) Ct /= 2;
+        blocks = (outer + Ct - 1) / Ct;
+    }
+    a.Ct = Ct;
+    const size_t smem = (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es;
+    if (smem > lds || !lds_fits(ctx, reinterpret_cast<const void*>(&line_fft_kernel<R>), smem)) {
+        ctx.err = "line longer than the LDS transform supports";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    if (blocks <= 0) return NDC_OK;
+    const double bytes = double(outer) * double(inner) * double(n) * double(es) * 2.0;
+    ctx.kbegin("fft_processor_lines", bytes, 5.0 * double(outer) * double(inner) * L * std::log2(double(std::max(2, L))));
+    hipLaunchKernelGGL((line_fft_kernel<R>), dim3(unsigned(blocks)), dim3(threads_for(int64_t(L) * Ct)), smem,
+                       ctx.stream(), a);
+    ctx.kend();
+    return check_launch(ctx, "processor line transform");
+}
+
+template <typename R>
+ndc_status transpose(LaunchContext& ctx, const void* in, void* out, int64_t A, int64_t B) {
+    if (A <= 0 || B <= 0) return NDC_OK;
+    const dim3 grid(unsigned((B + 31) / 32), unsigned((A + 31) / 32));
+    ctx.kbegin("fft_processor_transpose", 2.0 * double(A) * double(B) * double(sizeof(Cx<R>)), 0);
+    hipLaunchKernelGGL((transpose_kernel<R>), grid, dim3(32, 8), 0, ctx.stream(), static_cast<const Cx<R>*>(in),
+                       static_cast<Cx<R>*>(out), A, B);
+    ctx.kend();
+    return check_launch(ctx, "transpose");
+}
+
+// Forward N-d transform in the reference's layout: the natural spectrum with
+// its axes rotated left by one (processor/real.rs:105-155: r2c on the last
+// axis, then N-1 times "rotate axes right, transform the new last axis";
+// processor/complex.rs:33-115 likewise with c2c on every axis).
+template <typename R>
+ndc_status proc_forward(bool real, int N, const int64_t* shape, const void* in, void* out, LaunchContext& ctx) {
+    using C = Cx<R>;
+    const int64_t Fl = shape[N - 1];
+    const int64_t H = real ? Fl / 2 + 1 : Fl;
+    int64_t rows = 1;
+    for (int i = 0; i < N - 1; ++i) rows *= shape[i];
+    C* work = N == 1 ? static_cast<C*>(out) : static_cast<C*>(ctx.scratch(0, size_t(rows * H) * sizeof(C)));
+    if (!work) return NDC_ERR_OUT_OF_MEMORY;
+    ndc_status s = line_fft<R>(ctx, real ? 1 : 0, false, in, work, rows, int(Fl), 1, R(1));
+    if (s != NDC_OK) return s;
+    int64_t inner = H;
+    for (int ax = N - 2; ax >= 0; --ax) {
+        int64_t outer = 1;
+        for (int i = 0; i < ax; ++i) outer *= shape[i];
+        s = line_fft<R>(ctx, 0, false, work, work, outer, int(shape[ax]), inner, R(1));
+        if (s != NDC_OK) return s;
+        inner *= shape[ax];
+    }
+    if (N > 1) return transpose<R>(ctx, work, out, shape[0], rows / shape[0] * H);
+    return NDC_OK;
+}
+
+// Backward: undoes the rotation, inverse transforms, scales by 1/prod F
+// (processor/real.rs:233-281, complex.rs:134-145).  `fl` is the real length
+// of the last axis (the processor's rp_origin_len) for real output.
+template <typename R>
+ndc_status proc_backward(bool real, int N, const int64_t* sshape, int64_t fl, const void* in, void* out,
+                         LaunchContext& ctx) {
+    using C = Cx<R>;
+    // natural shape: the spectrum's last axis first
+    int64_t nat[kMaxRank];
+    nat[0] = sshape[N - 1];
+    for (int i = 0; i < N - 1; ++i) nat[i + 1] = sshape[i];
+    if (N == 1) nat[0] = sshape[0];
+    const int64_t H = nat[N - 1];
+    const int64_t Fl = real ? fl : H;
+    if (real && (Fl < 1 || H != Fl / 2 + 1)) {
+        ctx.err = "the spectrum's last axis does not match the real length of the last forward transform";
+        return NDC_ERR_INVALID_ARGUMENT;
+    }
+    double total = double(Fl);
+    int64_t rows = 1;
+    for (int i = 0; i < N - 1; ++i) {
+        rows *= nat[i];
+        total *= double(nat[i]);
+    }
+    const R scale = R(1.0 / total);
+    const C* src = static_cast<const C*>(in);
+    C* work = nullptr;
+    if (N > 1) {
+        work = static_cast<C*>(ctx.scratch(0, size_t(rows * H) * sizeof(C)));
+        if (!work) return NDC_ERR_OUT_OF_MEMORY;
+        ndc_status s = transpose<R>(ctx, in, work, rows / nat[0] * H, nat[0]);
+        if (s != NDC_OK) return s;
+        int64_t outer = 1;
+        for (int ax = 0; ax < N - 1; ++ax) {
+            int64_t inner = H;
+            for (int i = ax + 1; i < N - 1; ++i) inner *= nat[i];
+            s = line_fft<R>(ctx, 0, true, work, work, outer, int(nat[ax]), inner, R(1));
+            if (s != NDC_OK) return s;
+            outer *= nat[ax];
+        }
+        src = work;
+    }
+    return line_fft<R>(ctx, real ? 2 : 0, true, src, out, rows, int(Fl), 1, scale);
+}
+
 }  // namespace
 
 ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx) {
@@ -1460,6 +1758,16 @@ ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, Laun
         case NDC_C128: return run_fft<double, double2>(p, x, k, y, ctx);
         default: return NDC_ERR_UNSUPPORTED;
     }
+}
+
+ndc_status launch_fft_processor(int dtype, bool forward, int N, const int64_t* shape, int64_t origin_len,
+                                const void* in, void* out, LaunchContext& ctx) {
+    const bool real = dtype == NDC_F32 || dtype == NDC_F64;
+    const bool dbl = dtype == NDC_F64 || dtype == NDC_C128;
+    if (dbl) return forward ? proc_forward<double>(real, N, shape, in, out, ctx)
+                            : proc_backward<double>(real, N, shape, origin_len, in, out, ctx);
+    return forward ? proc_forward<float>(real, N, shape, in, out, ctx)
+                   : proc_backward<float>(real, N, shape, origin_len, in, out, ctx);
 }
 
 }  // namespace ndc

@@ -75,4 +75,12 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
 // FFT convolution (conv_fft/mod.rs:185-292).
 ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
 
+// The processor's plain N-d transforms (processor/real.rs, complex.rs) on
+// contiguous device arrays.  dtype F32/F64: real input (forward) or output
+// (backward); C64/C128: complex.  Forward: `shape` is the input's, the output
+// is the spectrum with axes rotated left by one.  Backward: `shape` is the
+// spectrum's; origin_len is the real last-axis length.
+ndc_status launch_fft_processor(int dtype, bool forward, int N, const int64_t* shape, int64_t origin_len,
+                                const void* in, void* out, LaunchContext& ctx);
+
 }  // namespace ndc

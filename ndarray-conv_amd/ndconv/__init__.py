@@ -344,13 +344,80 @@ def _desc_for_shapes(xshape, kshape, conv_mode, padding_mode, dilation=1, revers
 class FftProcessor:
     """Owns a library handle: HIP stream, device workspace, twiddle tables and
     plan caches.  Reusing one across calls is what conv_fft_with_processor is
-    for.  Not thread-safe (use one per thread, as &mut Processor)."""
+    for.  Not thread-safe (use one per thread, as &mut Processor).
 
-    def __init__(self):
+    forward/backward are Processor::forward/backward (processor/mod.rs:83-119):
+    `elem` is the processor's InElem (a real dtype for real.rs, a complex one
+    for complex.rs; by default the dtype of the first forward input)."""
+
+    def __init__(self, elem=None):
         lib = C.lib()
         h = ctypes.c_void_p()
         _check(lib.ndc_create(ctypes.byref(h)))
         self._h = h
+        self._elem = None if elem is None else np.dtype(elem)
+
+    def _transform(self, a, forward):
+        lib = C.lib()
+        h = self._h
+        torch_in = _is_torch(a)
+        dt = _np_dtype_of(a)
+        if forward:
+            if self._elem is None:
+                self._elem = dt
+            if dt != self._elem:
+                raise InvalidArgument(f"this processor transforms {self._elem}, not {dt}")
+            code = _NP_DTYPE.get(dt)
+        else:
+            if self._elem is None:
+                raise InvalidArgument("backward before any forward: the processor's element type is unknown")
+            if dt != np.dtype(np.complex64 if self._elem in (np.float32, np.complex64) else np.complex128):
+                raise InvalidArgument(f"a spectrum of {self._elem} is complex{64 if self._elem.itemsize <= 8 else 128}")
+            code = _NP_DTYPE.get(self._elem)
+        if code not in (C.F32, C.F64, C.C64, C.C128):
+            raise Unsupported(f"dtype {dt}")
+        shape = tuple(int(v) for v in a.shape)
+        n = len(shape)
+        real = self._elem.kind == "f"
+        if forward:
+            last = shape[-1] // 2 + 1 if real else shape[-1]
+            out_shape = shape[1:-1] + (last,) + shape[:1] if n > 1 else (last,)
+            out_dt = np.dtype(np.complex64 if code in (C.F32, C.C64) else np.complex128)
+        else:
+            nat = shape[-1:] + shape[:-1] if n > 1 else shape
+            if real:
+                o = ctypes.c_int64()
+                _check(lib.ndc_fft_origin_len(h, ctypes.byref(o)), h)
+                nat = nat[:-1] + (o.value,)
+            out_shape = nat
+            out_dt = self._elem
+        sh = (ctypes.c_int64 * n)(*shape)
+        fn = lib.ndc_fft_forward if forward else lib.ndc_fft_backward
+        if torch_in:
+            import torch
+            if not a.is_cuda or not a.is_contiguous():
+                raise InvalidArgument("torch inputs must be contiguous GPU tensors")
+            tdt = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64,
+                   np.dtype(np.complex64): torch.complex64, np.dtype(np.complex128): torch.complex128}[out_dt]
+            y = torch.empty(out_shape, dtype=tdt, device=a.device)
+            _check(lib.ndc_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream)), h)
+            _check(fn(h, code, n, sh, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(y.data_ptr()), 0), h)
+            return y
+        x = np.ascontiguousarray(a)
+        y = np.empty(out_shape, dtype=out_dt)
+        _check(lib.ndc_reset_stream(h), h)
+        _check(fn(h, code, n, sh, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data), C.FLAG_HOST_MEMORY), h)
+        return y
+
+    def forward(self, input, parallel=False):
+        """Processor::forward (processor/mod.rs:83-96): the N-d DFT with the axes
+        rotated left by one (real input: half spectrum on the input's last axis)."""
+        return self._transform(input, True)
+
+    def backward(self, input, parallel=False):
+        """Processor::backward (processor/mod.rs:98-119): the inverse of forward,
+        divided by the number of elements, in natural axis order."""
+        return self._transform(input, False)
 
     @property
     def handle(self):
@@ -371,9 +438,9 @@ class FftProcessor:
             pass
 
 
-def get_fft_processor():
-    """get_fft_processor (processor/mod.rs:71-73)."""
-    return FftProcessor()
+def get_fft_processor(elem=None):
+    """get_fft_processor (processor/mod.rs:71-73); `elem` is InElem."""
+    return FftProcessor(elem)
 
 
 _tls = threading.local()

@@ -114,6 +114,12 @@ def lib():
         f.argtypes = [H, ctypes.POINTER(ConvDesc), P, P, P]
     B = ctypes.c_void_p
     I64P = ctypes.POINTER(ctypes.c_int64)
+    for name in ("ndc_fft_forward", "ndc_fft_backward"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [H, ctypes.c_int32, ctypes.c_int32, I64P, P, P, ctypes.c_uint32]
+    L.ndc_fft_origin_len.restype = ctypes.c_int
+    L.ndc_fft_origin_len.argtypes = [H, I64P]
     L.ndc_batch_create.restype = ctypes.c_int
     L.ndc_batch_create.argtypes = [H, ctypes.POINTER(BatchDesc), ctypes.POINTER(B)]
     L.ndc_batch_destroy.restype = ctypes.c_int

@@ -30,6 +30,11 @@ reference's own golden vectors, by torch, or by mathematics) is listed here:
                                           (conv_fft/tests.rs:148-619) and a brute-force
                                           circular-convolution DFT on tiny inputs.  Complex
                                           results are pinned by the direct complex sum.
+  processor_forward / processor_backward  processor/real.rs:105-155,233-281, complex.rs:33-145
+                                          pinned: the literal permute-and-transform loop
+                                          (processor_forward_procedural), a brute-force DFT
+                                          on tiny inputs, and the reference's roundtrip cases
+                                          (real.rs:289-478, complex.rs:154-480)
 
 Representations (kept independent of the product's own API objects):
   conv mode    "full" | "same" | "valid" | ("custom", [p]*N, [s]*N)
@@ -503,3 +508,51 @@ def relerr(y, ref):
     m = float(np.max(np.abs(ref))) if ref.size else 0.0
     mx = float(np.max(d)) if d.size else 0.0
     return (mx / m if m > 0 else mx), mx
+
+
+# ----------------------------------------------------------------------------
+# Processor::forward / backward (processor/real.rs:105-155,233-281,
+# processor/complex.rs:33-145)
+# ----------------------------------------------------------------------------
+
+def processor_forward(x):
+    """The N-d DFT (forward, unnormalised) with its axes rotated left by one,
+    the layout the reference's processors return: the last axis is
+    transformed first (r2c for real input, F/2+1 bins), then N-1 times the
+    axes are rotated right and the new last axis is transformed
+    (real.rs:117-152, complex.rs:54-110).  Pinned against the procedural
+    restatement below and the reference's roundtrip tests."""
+    x = np.asarray(x)
+    X = np.fft.rfftn(x) if not np.iscomplexobj(x) else np.fft.fftn(x)
+    return np.moveaxis(X, 0, -1) if x.ndim > 1 else X
+
+
+def processor_forward_procedural(x):
+    """Literal restatement of the reference's loop: transform the last axis,
+    then N-1 times permute axes by rotate_right(1), copy contiguous, and
+    transform the new last axis."""
+    x = np.asarray(x)
+    n = x.ndim
+    out = np.fft.rfft(x, axis=-1) if not np.iscomplexobj(x) else np.fft.fft(x, axis=-1)
+    axes = list(range(n))
+    axes = axes[-1:] + axes[:-1]   # rotate_right(1)
+    for _ in range(n - 1):
+        out = np.ascontiguousarray(np.transpose(out, axes))
+        out = np.fft.fft(out, axis=-1)
+    return out
+
+
+def processor_backward(spec, real_len=None):
+    """Inverse of processor_forward divided by the number of output elements
+    (real.rs:233-281 with rp_origin_len = real_len, complex.rs:134-145).  For
+    real output the imaginary parts of the DC and Nyquist bins are ignored."""
+    spec = np.asarray(spec)
+    X = np.moveaxis(spec, -1, 0) if spec.ndim > 1 else spec
+    if real_len is None:
+        return np.fft.ifftn(X)
+    X = X.copy()
+    X = np.fft.ifftn(X, axes=list(range(X.ndim - 1))) if X.ndim > 1 else X
+    X[..., 0] = X[..., 0].real
+    if real_len % 2 == 0:
+        X[..., real_len // 2] = X[..., real_len // 2].real
+    return np.fft.irfft(X, n=real_len, axis=-1)

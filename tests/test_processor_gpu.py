@@ -1,0 +1,94 @@
+"""The FFT processor's forward/backward on the GPU (Processor::forward /
+backward, processor/mod.rs:83-119) against the oracle's restatement of the
+reference's layout: the N-d DFT with the axes rotated left by one, real input
+keeping F/2+1 bins of its last axis; backward returns natural order divided by
+the element count."""
+import numpy as np
+import pytest
+
+from oracle import ndconv_oracle as O
+
+import ndconv as nc
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.dtype(np.float32): 2e-6, np.dtype(np.float64): 1e-12, np.dtype(np.complex64): 2e-6,
+       np.dtype(np.complex128): 1e-12}
+
+SHAPES = [(6,), (7,), (425,), (5120,), (4, 6), (9, 35), (128, 145), (18, 128, 288), (3, 5, 7), (2, 3, 4, 5),
+          (256, 5120)]
+
+
+def make(rng, shape, dt):
+    dt = np.dtype(dt)
+    x = rng.standard_normal(shape)
+    if dt.kind == "c":
+        x = x + 1j * rng.standard_normal(shape)
+    return x.astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.complex64, np.complex128])
+def test_forward_backward_layout_and_values(gpu, dt):
+    rng = np.random.default_rng(5)
+    for shape in SHAPES:
+        if np.dtype(dt).itemsize >= 8 and np.dtype(dt) != np.dtype(np.complex64) and shape[-1] > 4096:
+            continue   # double lines of 5120 exceed one workgroup's LDS (see test_too_long_line_is_refused)
+        x = make(rng, shape, dt)
+        p = nc.get_fft_processor(dt)
+        spec = p.forward(x)
+        ref = O.processor_forward(x.astype(np.complex128 if np.dtype(dt).kind == "c" else np.float64))
+        assert spec.shape == ref.shape, (shape, spec.shape, ref.shape)
+        e = O.relerr(spec, ref)[0]
+        assert e <= TOL[np.dtype(dt)] * (10 if len(shape) > 2 else 1), (shape, e)
+        back = p.backward(spec)
+        assert back.shape == x.shape and back.dtype == x.dtype
+        e2 = O.relerr(back, x)[0]
+        assert e2 <= 5 * TOL[np.dtype(dt)] * (10 if len(shape) > 2 else 1), (shape, e2)
+        p.close()
+
+
+def test_reference_roundtrip_cases_on_gpu(gpu):
+    # real.rs roundtrip tests: 1e-10 for f64 inputs
+    cases = [np.array([1.0, 2, 3, 4, 5, 6]), np.array([1.0, 2]), np.array([1.0, 2, 3]),
+             np.array([1.0, 2, 3, 4, 5]), np.fromfunction(lambda i, j: (i + j) % 17, (32, 32)),
+             np.fromfunction(lambda i, j, k: (i + j + k) % 13, (8, 8, 8))]
+    p = nc.get_fft_processor(np.float64)
+    for x in cases:
+        back = p.backward(p.forward(x))
+        assert np.max(np.abs(back - x)) < 1e-10
+
+
+def test_backward_uses_the_last_forward_length(gpu):
+    # rp_origin_len: the real length recorded by the last forward (real.rs:108,263)
+    rng = np.random.default_rng(6)
+    p = nc.get_fft_processor(np.float32)
+    x6 = rng.standard_normal((3, 6)).astype(np.float32)
+    x7 = rng.standard_normal((3, 7)).astype(np.float32)
+    s6 = p.forward(x6)
+    s7 = p.forward(x7)   # both have 4 bins: the last forward decides the output length
+    assert s6.shape == s7.shape == (4, 3)
+    assert p.backward(s6).shape == (3, 7)
+    assert O.relerr(p.backward(s7), x7)[0] <= 1e-5
+    with pytest.raises(nc.InvalidArgument):
+        nc.get_fft_processor(np.float32).backward(s7)   # no forward on this processor yet
+    bad = np.zeros((6, 3), np.complex64)                # 6 bins do not belong to length 7
+    with pytest.raises(nc.InvalidArgument):
+        p.backward(bad)
+
+
+def test_torch_device_tensors(gpu):
+    import torch
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((64, 300)).astype(np.float32)
+    p = nc.get_fft_processor(np.float32)
+    spec = p.forward(torch.from_numpy(x).cuda())
+    ref = O.processor_forward(x.astype(np.float64))
+    assert O.relerr(spec.cpu().numpy(), ref)[0] <= 2e-6
+    back = p.backward(spec)
+    assert O.relerr(back.cpu().numpy(), x)[0] <= 1e-5
+
+
+def test_too_long_line_is_refused(gpu):
+    p = nc.get_fft_processor(np.complex128)
+    with pytest.raises(nc.Unsupported):
+        p.forward(np.zeros(16384, np.complex128))
