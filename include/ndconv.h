@@ -195,6 +195,20 @@ ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, cons
  * threading and planner reuse, which the handle subsumes. */
 ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
 
+/* A kernel spectrum kept for repeated convolutions of same-shaped data with
+ * one kernel: the reference's sketched ConvFFTExt::conv_fft_bake /
+ * conv_fft_with_baked and Baked (conv_fft/mod.rs:22-42,175-182,306-392,
+ * commented out there).  ndc_conv_fft_bake validates `desc` exactly as
+ * ndc_conv_fft does, transforms the kernel once and keeps its spectrum (plans
+ * whose last axis is too long for one workgroup keep a copy of the kernel
+ * instead).  ndc_conv_fft_with_baked convolves x, which has desc's shape,
+ * strides and flags, and writes y; it equals ndc_conv_fft on the same
+ * arguments.  The baked object uses h and must be destroyed before it. */
+typedef struct ndc_baked_s* ndc_baked;
+ndc_status ndc_conv_fft_bake(ndc_handle h, const ndc_conv_desc* desc, const void* k, ndc_baked* out);
+ndc_status ndc_conv_fft_with_baked(ndc_baked b, const void* x, void* y);
+ndc_status ndc_baked_destroy(ndc_baked b);
+
 /* The FFT processor's plain transforms (Processor::forward / backward,
  * processor/mod.rs:83-119; real.rs:105-155,233-281; complex.rs:33-145) on a
  * C-contiguous array.  The handle is the processor.

@@ -253,6 +253,111 @@ ndc_status run(ndc_handle hh, const ndc_conv_desc* d, int op, const void* x, con
     return NDC_OK;
 }
 
+// A kept kernel spectrum (the reference's sketched Baked form,
+// conv_fft/mod.rs:22-42,306-392): the plan of one data shape and one kernel.
+struct Baked {
+    ndc_handle h = nullptr;
+    Plan p;
+    DevBuf spec;    // the kernel's spectrum, or empty when the rows are too long to keep one
+    DevBuf kcopy;   // the kernel itself for those plans (same element offsets as the view)
+    int64_t k_lo = 0;
+    ~Baked() {
+        if (spec.ptr) (void)hipFree(spec.ptr);
+        if (kcopy.ptr) (void)hipFree(kcopy.ptr);
+    }
+};
+
+ndc_status bake(ndc_handle hh, const ndc_conv_desc* d, const void* k, Baked** out) {
+    if (!hh || !d || !out) return NDC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    Handle& h = *reinterpret_cast<Handle*>(hh);
+    auto b = std::make_unique<Baked>();
+    b->h = hh;
+    std::string err;
+    ndc_status s = make_plan(d, NDC_OP_CONV_FFT, &b->p, &err, true);
+    if (s != NDC_OK) return set_err(hh, s, err);
+    if (!k) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "null kernel pointer");
+    if (hipSetDevice(h.device) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "hipSetDevice failed");
+    const Plan& p = b->p;
+    const size_t es = dtype_size(p.dtype);
+    const bool host = (p.flags & NDC_FLAG_HOST_MEMORY) != 0;
+    // the kernel on the device, at the view's element offsets
+    const size_t kb = size_t(p.k_hi - p.k_lo + 1) * es;
+    const void* dk = k;
+    const size_t sb = fft_kernel_spectrum_bytes(p, h.max_lds());
+    if (host || sb == 0) {
+        if (hipMalloc(&b->kcopy.ptr, kb) != hipSuccess) return set_err(hh, NDC_ERR_OUT_OF_MEMORY, "kernel copy allocation failed");
+        b->kcopy.bytes = kb;
+        const char* src = static_cast<const char*>(k) + p.k_lo * int64_t(es);
+        if (hipMemcpyAsync(b->kcopy.ptr, src, kb, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, h.stream()) !=
+            hipSuccess)
+            return set_err(hh, NDC_ERR_DEVICE, "kernel copy failed");
+        b->k_lo = p.k_lo;
+        dk = static_cast<const char*>(b->kcopy.ptr) - p.k_lo * int64_t(es);
+    }
+    if (sb > 0) {
+        if (hipMalloc(&b->spec.ptr, sb) != hipSuccess) return set_err(hh, NDC_ERR_OUT_OF_MEMORY, "spectrum allocation failed");
+        b->spec.bytes = sb;
+        KernelSpectrum ks;
+        ks.spec = b->spec.ptr;
+        ks.bake = true;
+        h.err.clear();
+        s = launch_fft(p, nullptr, dk, nullptr, h, &ks);
+        if (s != NDC_OK) return set_err(hh, s, h.err);
+        if (b->kcopy.ptr && sb > 0) {
+            // the spectrum is all that later calls read
+            if (hipStreamSynchronize(h.stream()) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "synchronize failed");
+            (void)hipFree(b->kcopy.ptr);
+            b->kcopy.ptr = nullptr;
+        }
+    }
+    if (host && hipStreamSynchronize(h.stream()) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "synchronize failed");
+    *out = b.release();
+    return NDC_OK;
+}
+
+ndc_status run_baked(Baked& b, const void* x, void* y) {
+    ndc_handle hh = b.h;
+    Handle& h = *reinterpret_cast<Handle*>(hh);
+    const Plan& p = b.p;
+    if (!x || !y) return set_err(hh, NDC_ERR_INVALID_ARGUMENT, "null data pointer");
+    if (hipSetDevice(h.device) != hipSuccess) return set_err(hh, NDC_ERR_DEVICE, "hipSetDevice failed");
+    const size_t es = dtype_size(p.dtype);
+    const bool host = (p.flags & NDC_FLAG_HOST_MEMORY) != 0;
+    const void* dx = x;
+    void* dy = y;
+    if (host) {
+        const size_t xb = size_t(p.x_hi - p.x_lo + 1) * es;
+        const size_t yb = size_t(p.y_numel) * es;
+        char* bx = static_cast<char*>(h.grow(h.io[0], xb));
+        char* by = static_cast<char*>(h.grow(h.io[2], yb ? yb : es));
+        if (!bx || !by) return set_err(hh, NDC_ERR_OUT_OF_MEMORY, "device buffer allocation failed");
+        if (hipMemcpyAsync(bx, static_cast<const char*>(x) + p.x_lo * int64_t(es), xb, hipMemcpyHostToDevice,
+                           h.stream()) != hipSuccess)
+            return set_err(hh, NDC_ERR_DEVICE, "host to device copy failed");
+        dx = bx - p.x_lo * int64_t(es);
+        dy = by;
+    }
+    h.err.clear();
+    ndc_status s;
+    if (b.spec.ptr) {
+        KernelSpectrum ks;
+        ks.spec = b.spec.ptr;
+        ks.bake = false;
+        s = launch_fft(p, dx, nullptr, dy, h, &ks);
+    } else {
+        const void* dk = static_cast<const char*>(b.kcopy.ptr) - b.k_lo * int64_t(es);
+        s = launch_fft(p, dx, dk, dy, h);
+    }
+    if (s != NDC_OK) return set_err(hh, s, h.err);
+    if (host) {
+        if (hipMemcpyAsync(y, dy, size_t(p.y_numel) * es, hipMemcpyDeviceToHost, h.stream()) != hipSuccess ||
+            hipStreamSynchronize(h.stream()) != hipSuccess)
+            return set_err(hh, NDC_ERR_DEVICE, "device to host copy failed");
+    }
+    return NDC_OK;
+}
+
 // The processor's forward/backward (processor/mod.rs:83-119).
 ndc_status run_processor(ndc_handle hh, bool forward, int32_t dtype, int32_t rank, const int64_t* shape,
                          const void* x, void* y, uint32_t flags) {
@@ -449,6 +554,27 @@ ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, cons
 
 ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y) {
     return ndc::run(h, desc, NDC_OP_CONV_FFT, x, k, y);
+}
+
+ndc_status ndc_conv_fft_bake(ndc_handle h, const ndc_conv_desc* desc, const void* k, ndc_baked* out) {
+    if (!out) return NDC_ERR_INVALID_ARGUMENT;
+    ndc::Baked* b = nullptr;
+    const ndc_status s = ndc::bake(h, desc, k, &b);
+    *out = reinterpret_cast<ndc_baked>(b);
+    return s;
+}
+
+ndc_status ndc_conv_fft_with_baked(ndc_baked b, const void* x, void* y) {
+    if (!b) return NDC_ERR_INVALID_ARGUMENT;
+    return ndc::run_baked(*reinterpret_cast<ndc::Baked*>(b), x, y);
+}
+
+ndc_status ndc_baked_destroy(ndc_baked b) {
+    if (!b) return NDC_ERR_INVALID_ARGUMENT;
+    ndc::Baked* p = reinterpret_cast<ndc::Baked*>(b);
+    (void)hipStreamSynchronize(reinterpret_cast<Handle*>(p->h)->stream());
+    delete p;
+    return NDC_OK;
 }
 
 ndc_status ndc_fft_forward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* shape, const void* x, void* y,

@@ -444,7 +444,8 @@ struct ColsArgs {
     int32_t d;                       // transformed axis
     int32_t valid[2];                // rows of axis d holding data (rest are zero)
     AxisSel outsel;                  // rows of axis d to write back
-    int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse
+    int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse,
+                                     // 3 fused with the kernel spectrum already transformed
     LinePlan<R> pl;
 };
 
@@ -515,6 +516,18 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         C* dres = cd ? kfree : b0;
         C* dfree = cd ? b0 : kfree;
         for (int it = threadIdx.x; it < L * Ct; it += T) dres[it] = cmul(dres[it], kres[it]);
+        __syncthreads();
+        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true) ? dfree : dres;
+    } else if (a.mode == 3) {
+        // kernel columns already transformed (kept spectrum): data forward,
+        // product, inverse
+        load(a.spec[1], b1, L);
+        load(a.spec[0], b0, a.valid[0]);
+        __syncthreads();
+        const int cd = fft_lines<R, true>(b0, b2, a.pl, tw, Ct, 1, Ct, false);
+        C* dres = cd ? b2 : b0;
+        C* dfree = cd ? b0 : b2;
+        for (int it = threadIdx.x; it < L * Ct; it += T) dres[it] = cmul(dres[it], b1[it]);
         __syncthreads();
         res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true) ? dfree : dres;
     } else {
@@ -1029,7 +1042,7 @@ int threads_for(int64_t points) {
 }
 
 template <typename R, typename E>
-ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, LaunchContext& ctx) {
+ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, LaunchContext& ctx, KernelSpectrum* ks) {
     using C = Cx<R>;
     constexpr bool kComplex = std::is_same<E, C>::value;
     const int N = p.N;
@@ -1100,8 +1113,12 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     E* y = static_cast<E*>(yv);
     const double esz = double(sizeof(E));
 
+    if (ks && big) {
+        ctx.err = "a kept kernel spectrum needs LDS-resident rows";
+        return NDC_ERR_UNSUPPORTED;
+    }
     // rank 1, line fits three buffers: one fused launch
-    if (N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * L) * es <= lds) {
+    if (!ks && N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * L) * es <= lds) {
         Fused1dArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
         a.pad = pad;
@@ -1129,7 +1146,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     for (int e = last - 2; e >= 0; --e) sstr[e] = sstr[e + 1] * p.fft[e + 1];
     const int64_t spec_elems = N >= 2 ? sstr[0] * p.fft[0] : Hp;
     C* Sx = static_cast<C*>(ctx.scratch(0, size_t(spec_elems) * es));
-    C* Sk = static_cast<C*>(ctx.scratch(1, size_t(spec_elems) * es));
+    C* Sk = ks ? static_cast<C*>(ks->spec) : static_cast<C*>(ctx.scratch(1, size_t(spec_elems) * es));
     if (!Sx || !Sk) {
         ctx.err = "spectrum workspace allocation failed";
         return NDC_ERR_OUT_OF_MEMORY;
@@ -1142,7 +1159,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         return lpb;
     };
 
-    auto rows_fwd = [&](bool data_job) -> ndc_status {
+    auto rows_fwd = [&](bool data_job, bool kernel_job) -> ndc_status {
         RowsFwdArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
         a.pad = pad;
@@ -1155,7 +1172,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
             a.cnt[1][e] = p.kdil[e];
         }
         a.nlines[0] = data_job ? 1 : 0;
-        a.nlines[1] = 1;
+        a.nlines[1] = kernel_job ? 1 : 0;
         for (int e = 0; e < last; ++e) {
             a.nlines[0] *= a.cnt[0][e];
             a.nlines[1] *= a.cnt[1][e];
@@ -1172,7 +1189,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const int64_t nb0 = (a.nlines[0] + lpb - 1) / lpb;
         const int64_t nb1 = (a.nlines[1] + lpb - 1) / lpb;
         a.nblocks0 = int32_t(nb0);
-        const double bytes = esz * double((data_job ? p.x_numel : 0) + p.k_numel) +
+        const double bytes = esz * double((data_job ? p.x_numel : 0) + (kernel_job ? p.k_numel : 0)) +
                              double(es) * double(a.nlines[0] + a.nlines[1]) * H;
         ctx.kbegin("fft_rows_fwd", bytes, 5.0 * double(a.nlines[0] + a.nlines[1]) * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_fwd_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
@@ -1184,8 +1201,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
 
     // rank 1, line fits two buffers: kernel spectrum first, then the fused data pass
     if (N == 1 && !big) {
-        ndc_status s = rows_fwd(false);
-        if (s != NDC_OK) return s;
+        if (!ks || ks->bake) {
+            ndc_status s = rows_fwd(false, true);
+            if (s != NDC_OK || ks) return s;
+        }
         Fused1dArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
         a.pad = pad;
@@ -1310,12 +1329,13 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     }
 
     // ---- pass 1: last axis forward
+    const bool data_job = !(ks && ks->bake), kernel_job = !(ks && !ks->bake);
     if (big) {
         ndc_status s = big_col(false);
         if (s == NDC_OK) s = big_row(0);
         if (s != NDC_OK) return s;
     } else {
-        ndc_status s = rows_fwd(true);
+        ndc_status s = rows_fwd(data_job, kernel_job);
         if (s != NDC_OK) return s;
     }
 
@@ -1355,14 +1375,16 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.valid[1] = int(p.kdil[d]);
         if (cmode == 0) a.outsel = AxisSel{0, 1, Ld};
         else a.outsel = AxisSel{p.kdil[d] - 1, p.stride[d], p.out[d]};
-        const int nbuf = cmode == 1 ? 3 : 2;
+        if (cmode == 0 && !data_job) a.nlines_outer[0] = 0;
+        if (cmode == 0 && !kernel_job) a.nlines_outer[1] = 0;
+        const int nbuf = (cmode == 1 || cmode == 3) ? 3 : 2;
         // tile width: enough lines for coalescing, enough blocks to fill the chip
         int Ct = 16;
         auto smem_of = [&](int ct) { return (size_t(a.pl.ntw) + size_t(nbuf) * Ld * ct) * es; };
         while (Ct > 1 && smem_of(Ct) > std::min<size_t>(lds, 96 * 1024)) Ct /= 2;
         // 8 columns = 64-byte segments per row (f32): below that, partial-line
         // reads and writes cost more than the extra blocks gain
-        while (Ct > 8 && a.nlines_outer[0] * ((H + Ct - 1) / Ct) < 2 * ctx.cu_count()) Ct /= 2;
+        while (Ct > 8 && (a.nlines_outer[0] + a.nlines_outer[1]) * ((H + Ct - 1) / Ct) < 2 * ctx.cu_count()) Ct /= 2;
         const size_t smem = smem_of(Ct);
         if (smem > lds) {
             ctx.err = "FFT length of a column axis exceeds the LDS-resident line size";
@@ -1378,9 +1400,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         double bytes = 0, nfft = 0;
         if (cmode == 0) { bytes = (lines0 * (a.valid[0] + Ld) + lines1 * (a.valid[1] + Ld)); nfft = lines0 + lines1; }
         else if (cmode == 1) { bytes = lines0 * (a.valid[0] + a.valid[1] + double(a.outsel.count)); nfft = 3 * lines0; }
+        else if (cmode == 3) { bytes = lines0 * (a.valid[0] + Ld + double(a.outsel.count)); nfft = 2 * lines0; }
         else { bytes = lines0 * (Ld + double(a.outsel.count)); nfft = lines0; }
         bytes *= double(es);
-        static const char* names[3] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv"};
+        static const char* names[4] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv", "fft_cols_fused_kept"};
         ctx.kbegin(names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&cols_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
         hipLaunchKernelGGL((cols_kernel<R>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(Ld) * Ct / 2)), smem,
@@ -1394,9 +1417,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         ndc_status s = col_pass(d, 0);
         if (s != NDC_OK) return s;
     }
-    // ---- pass 3: axis 0 fused
+    // ---- pass 3: axis 0 fused (or, when baking, the kernel's axis-0 forward)
+    if (ks && ks->bake) return col_pass(0, 0);
     {
-        ndc_status s = col_pass(0, 1);
+        ndc_status s = col_pass(0, ks ? 3 : 1);
         if (s != NDC_OK) return s;
     }
     // ---- pass 4: axes 1 .. N-2 inverse
@@ -1750,14 +1774,29 @@ ndc_status proc_backward(bool real, int N, const int64_t* sshape, int64_t fl, co
 
 }  // namespace
 
-ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx) {
+ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx, KernelSpectrum* ks) {
     switch (p.dtype) {
-        case NDC_F32: return run_fft<float, float>(p, x, k, y, ctx);
-        case NDC_F64: return run_fft<double, double>(p, x, k, y, ctx);
-        case NDC_C64: return run_fft<float, float2>(p, x, k, y, ctx);
-        case NDC_C128: return run_fft<double, double2>(p, x, k, y, ctx);
+        case NDC_F32: return run_fft<float, float>(p, x, k, y, ctx, ks);
+        case NDC_F64: return run_fft<double, double>(p, x, k, y, ctx, ks);
+        case NDC_C64: return run_fft<float, float2>(p, x, k, y, ctx, ks);
+        case NDC_C128: return run_fft<double, double2>(p, x, k, y, ctx, ks);
         default: return NDC_ERR_UNSUPPORTED;
     }
+}
+
+size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds) {
+    const bool cplx = p.dtype == NDC_C64 || p.dtype == NDC_C128;
+    const size_t es = (p.dtype == NDC_F64 || p.dtype == NDC_C128) ? 16 : 8;
+    const int N = p.N, last = N - 1;
+    const int64_t Fl = p.fft[last];
+    const int mode = cplx ? 2 : (Fl % 2 == 0 ? 0 : 1);
+    const int64_t L = mode == 0 ? Fl / 2 : Fl;
+    if ((size_t(ntw_of(int(L), mode == 0 ? int(Fl) : 0)) + size_t(2) * L) * es > lds) return 0;   // four-step rows
+    const int64_t H = cplx ? Fl : Fl / 2 + 1;
+    const int64_t Hp = (H + 7) / 8 * 8;
+    int64_t n = Hp;
+    for (int e = 0; e < last; ++e) n *= p.fft[e];
+    return size_t(n) * es;
 }
 
 ndc_status launch_fft_processor(int dtype, bool forward, int N, const int64_t* shape, int64_t origin_len,

@@ -72,8 +72,22 @@ ndc_status launch_direct(const Plan& p, const void* x, const void* k, void* y, L
 // (p.algo), FP32 accumulation.
 ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
 
+// A kernel spectrum kept across calls (the form the reference sketches as
+// "Baked", conv_fft/mod.rs:22-42,306-392): with bake set, launch_fft only
+// computes the kernel's spectrum into `spec`; otherwise it reads it from there
+// and transforms only the data.
+struct KernelSpectrum {
+    void* spec = nullptr;
+    bool bake = false;
+};
+
 // FFT convolution (conv_fft/mod.rs:185-292).
-ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx);
+ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, LaunchContext& ctx,
+                      KernelSpectrum* ks = nullptr);
+
+// Bytes of a kept kernel spectrum for the plan, or 0 when its rows are too
+// long for one workgroup (those plans recompute the kernel spectrum per call).
+size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds);
 
 // The processor's plain N-d transforms (processor/real.rs, complex.rs) on
 // contiguous device arrays.  dtype F32/F64: real input (forward) or output

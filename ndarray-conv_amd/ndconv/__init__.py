@@ -32,7 +32,8 @@ from . import _lib as C
 __all__ = ["ConvMode", "PaddingMode", "BorderType", "KernelWithDilation", "with_dilation", "reverse",
            "no_reverse", "conv", "conv_fft", "conv_fft_with_processor", "conv_fft_par", "get_fft_processor",
            "FftProcessor", "plan", "border_map", "Error", "DataShape", "KernelShape", "MismatchShape",
-           "InvalidArgument", "Unsupported", "DeviceError", "BatchPlan", "conv_fft_batch", "batch_border_map"]
+           "InvalidArgument", "Unsupported", "DeviceError", "BatchPlan", "conv_fft_batch", "batch_border_map",
+           "Baked", "conv_fft_bake", "conv_fft_with_baked"]
 
 
 # ---------------------------------------------------------------------------
@@ -544,6 +545,89 @@ def conv_fft(data, kernel, conv_mode, padding_mode):
 def conv_fft_with_processor(data, kernel, conv_mode, padding_mode, fft_processor):
     """ConvFFTExt::conv_fft_with_processor (conv_fft/mod.rs:151-157)."""
     return _run(C.OP_CONV_FFT, data, kernel, conv_mode, padding_mode, fft_processor)
+
+
+class Baked:
+    """A kernel spectrum kept for repeated conv_fft of same-shaped data with
+    one kernel: the reference's sketched Baked (conv_fft/mod.rs:22-42,306-392).
+    The data given to conv_fft_bake fixes the shape, strides, dtype and memory
+    kind (numpy or torch GPU) that conv_fft_with_baked accepts."""
+
+    def __init__(self, data, kernel, conv_mode, padding_mode, processor=None):
+        lib = C.lib()
+        self._proc = processor or _default_processor()
+        kwd = _into_kwd(kernel)
+        self._torch = _is_torch(data) or _is_torch(kwd.kernel)
+        if self._torch:
+            if not (_is_torch(data) and _is_torch(kwd.kernel)) or not data.is_cuda or not kwd.kernel.is_cuda:
+                raise InvalidArgument("torch inputs must both be GPU tensors")
+            x, k = data, kwd.kernel
+        else:
+            x, k = np.asarray(data), np.asarray(kwd.kernel)
+            self._k = k   # keeps the host kernel alive for the duration of the bake
+        dt = _np_dtype_of(x)
+        if dt != _np_dtype_of(k):
+            raise InvalidArgument("data and kernel dtypes differ")
+        if dt not in _NP_DTYPE:
+            raise Unsupported(f"dtype {dt}")
+        d = C.ConvDesc()
+        _fill_desc(d, x, KernelWithDilation(k, kwd.dilation, kwd.reverse_flag), conv_mode, padding_mode, _NP_DTYPE[dt],
+                   host=not self._torch)
+        info = C.PlanInfo()
+        _check(lib.ndc_plan(ctypes.byref(d), C.OP_CONV_FFT, ctypes.byref(info)))
+        self.out_shape = tuple(int(info.out_shape[i]) for i in range(len(x.shape)))
+        self._sig = (tuple(x.shape), tuple(x.stride() if self._torch else x.strides), dt)
+        self._dt = dt
+        h = self._proc.handle
+        b = ctypes.c_void_p()
+        if self._torch:
+            import torch
+            _check(lib.ndc_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)), h)
+            _check(lib.ndc_conv_fft_bake(h, ctypes.byref(d), ctypes.c_void_p(k.data_ptr()), ctypes.byref(b)), h)
+        else:
+            _check(lib.ndc_reset_stream(h), h)
+            _check(lib.ndc_conv_fft_bake(h, ctypes.byref(d), ctypes.c_void_p(k.ctypes.data), ctypes.byref(b)), h)
+        self._b = b
+
+    def conv(self, data):
+        lib = C.lib()
+        h = self._proc.handle
+        if _is_torch(data) != self._torch:
+            raise InvalidArgument("data must be of the memory kind the kernel was baked for")
+        x = data if self._torch else np.asarray(data)
+        if (tuple(x.shape), tuple(x.stride() if self._torch else x.strides), _np_dtype_of(x)) != self._sig:
+            raise InvalidArgument("data must have the shape, strides and dtype the kernel was baked for")
+        if self._torch:
+            import torch
+            y = torch.empty(self.out_shape, dtype=x.dtype, device=x.device)
+            _check(lib.ndc_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)), h)
+            _check(lib.ndc_conv_fft_with_baked(self._b, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr())), h)
+            return y
+        y = np.empty(self.out_shape, dtype=self._dt)
+        _check(lib.ndc_reset_stream(h), h)
+        _check(lib.ndc_conv_fft_with_baked(self._b, ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(y.ctypes.data)), h)
+        return y
+
+    def close(self):
+        if getattr(self, "_b", None):
+            C.lib().ndc_baked_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def conv_fft_bake(data, kernel, conv_mode, padding_mode):
+    """ConvFFTExt::conv_fft_bake as sketched in conv_fft/mod.rs:175-182,306-360."""
+    return Baked(data, kernel, conv_mode, padding_mode)
+
+
+def conv_fft_with_baked(data, baked):
+    """ConvFFTExt::conv_fft_with_baked as sketched in conv_fft/mod.rs:362-392."""
+    return baked.conv(data)
 
 
 def conv_fft_par(data, kernel, conv_mode, padding_mode):

@@ -118,6 +118,12 @@ def lib():
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [H, ctypes.c_int32, ctypes.c_int32, I64P, P, P, ctypes.c_uint32]
+    L.ndc_conv_fft_bake.restype = ctypes.c_int
+    L.ndc_conv_fft_bake.argtypes = [H, ctypes.POINTER(ConvDesc), P, ctypes.POINTER(B)]
+    L.ndc_conv_fft_with_baked.restype = ctypes.c_int
+    L.ndc_conv_fft_with_baked.argtypes = [B, P, P]
+    L.ndc_baked_destroy.restype = ctypes.c_int
+    L.ndc_baked_destroy.argtypes = [B]
     L.ndc_fft_origin_len.restype = ctypes.c_int
     L.ndc_fft_origin_len.argtypes = [H, I64P]
     L.ndc_batch_create.restype = ctypes.c_int
