@@ -210,7 +210,7 @@ ndc_status ndc_conv_fft_with_baked(ndc_baked b, const void* x, void* y);
 ndc_status ndc_baked_destroy(ndc_baked b);
 
 /* The FFT processor's plain transforms (Processor::forward / backward,
- * processor/mod.rs:83-119; real.rs:105-155,233-281; complex.rs:33-145) on a
+ * processor/mod.rs:79-121; real.rs:105-155,233-281; complex.rs:33-145) on a
  * C-contiguous array.  The handle is the processor.
  *  - dtype NDC_F32/NDC_F64: real input (forward) or real output (backward);
  *    NDC_C64/NDC_C128: complex both ways.

@@ -358,7 +358,7 @@ ndc_status run_baked(Baked& b, const void* x, void* y) {
     return NDC_OK;
 }
 
-// The processor's forward/backward (processor/mod.rs:83-119).
+// The processor's forward/backward (processor/mod.rs:79-121).
 ndc_status run_processor(ndc_handle hh, bool forward, int32_t dtype, int32_t rank, const int64_t* shape,
                          const void* x, void* y, uint32_t flags) {
     if (!hh) return NDC_ERR_INVALID_ARGUMENT;

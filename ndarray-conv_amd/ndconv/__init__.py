@@ -347,7 +347,7 @@ class FftProcessor:
     plan caches.  Reusing one across calls is what conv_fft_with_processor is
     for.  Not thread-safe (use one per thread, as &mut Processor).
 
-    forward/backward are Processor::forward/backward (processor/mod.rs:83-119):
+    forward/backward are Processor::forward/backward (processor/mod.rs:79-121):
     `elem` is the processor's InElem (a real dtype for real.rs, a complex one
     for complex.rs; by default the dtype of the first forward input)."""
 

@@ -1,5 +1,5 @@
 """The FFT processor's forward/backward on the GPU (Processor::forward /
-backward, processor/mod.rs:83-119) against the oracle's restatement of the
+backward, processor/mod.rs:79-121) against the oracle's restatement of the
 reference's layout: the N-d DFT with the axes rotated left by one, real input
 keeping F/2+1 bins of its last axis; backward returns natural order divided by
 the element count."""
