@@ -706,7 +706,7 @@ ndc_status ndc_batch_border_map(const ndc_batch_desc* desc, int64_t pair, int32_
     if (len != p.pds[0]) return NDC_ERR_INVALID_ARGUMENT;
     const int32_t front = ndc::side1d(p.side[0][0].tag), back = ndc::side1d(p.side[0][1].tag);
     for (int64_t q = 0; q < len; ++q)
-        codes[q] = int32_t(ndc::resolve1d(q, p.x_shape[0], p.pad[0][0], front, back));
+        codes[q] = int32_t(ndc::resolve1d<int64_t>(q, p.x_shape[0], p.pad[0][0], front, back));
     return NDC_OK;
 }
 

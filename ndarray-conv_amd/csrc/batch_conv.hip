@@ -184,7 +184,7 @@ __device__ __forceinline__ float pad_value(const BatchArgs& a, const LineCtx& c,
     if (P >= c.pds) return 0.f;
     const int64_t r = P - c.p0;
     if (r >= 0 && r < c.n) return a.x[c.x_off + r];
-    const int64_t s = resolve1d(P, c.n, c.p0, a.front, a.back);
+    const int64_t s = resolve1d<int64_t>(P, c.n, c.p0, a.front, a.back);
     if (s >= 0) return a.x[c.x_off + s];
     if (s == kConstFront) return a.cfront;
     if (s == kConstBack) return a.cback;

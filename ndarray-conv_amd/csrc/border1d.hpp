@@ -28,29 +28,32 @@ NDC_HD int32_t side1d(int32_t border_tag) {
     }
 }
 
-// Front pad cell p (< p0) after the front fill.
-NDC_HD int64_t resolve_front(int64_t p, int64_t n, int64_t p0, int32_t front) {
+// Front pad cell p (< p0) after the front fill.  I is the index type: int64_t
+// on the host and for batches, int32_t inside the LDS-resident FFT passes.
+template <typename I>
+NDC_HD I resolve_front(I p, I n, I p0, int32_t front) {
     switch (front) {
         case kS1Const: return kConstFront;
         case kS1Replicate: return 0;
         case kS1Reflect: {
             // src = 2 p0 - p lies after the interior start; the back pad is
             // not written yet during the front fill
-            const int64_t src = 2 * p0 - p;
-            return src < p0 + n ? src - p0 : int64_t(kUnwritten);
+            const I src = 2 * p0 - p;
+            return src < p0 + n ? src - p0 : I(kUnwritten);
         }
         default: {
             // circular: src = n + p; a front cell not written yet keeps the
             // initial value
-            const int64_t src = n + p;
-            return src < p0 ? int64_t(kUnwritten) : src - p0;
+            const I src = n + p;
+            return src < p0 ? I(kUnwritten) : src - p0;
         }
     }
 }
 
 // Source of padded position p in [0, n + p0 + p1): an index into x (>= 0) or
 // kConstFront / kConstBack / kUnwritten.
-NDC_HD int64_t resolve1d(int64_t p, int64_t n, int64_t p0, int32_t front, int32_t back) {
+template <typename I>
+NDC_HD I resolve1d(I p, I n, I p0, int32_t front, int32_t back) {
     if (p < p0) return resolve_front(p, n, p0, front);
     if (p < p0 + n) return p - p0;
     switch (back) {
@@ -59,7 +62,7 @@ NDC_HD int64_t resolve1d(int64_t p, int64_t n, int64_t p0, int32_t front, int32_
         case kS1Reflect: {
             // src = 2 (p0 + n - 1) - p; reaching into the front pad reads the
             // value the front fill left there
-            const int64_t src = 2 * (p0 + n - 1) - p;
+            const I src = 2 * (p0 + n - 1) - p;
             return src >= p0 ? src - p0 : resolve_front(src, n, p0, front);
         }
         default:

@@ -2,7 +2,8 @@
 //
 // The padded data and the scattered kernel are never materialised: the first
 // pass reads x through the per-axis border maps (conv_fft/padding.rs:30-62,
-// padding/half_dim.rs) and k through the dilation/reversal scatter rule
+// padding/half_dim.rs), evaluated in closed form per cell (border1d.hpp),
+// and k through the dilation/reversal scatter rule
 // (conv_fft/padding.rs:78-111) while loading rows into LDS.  Transforms are
 // mixed-radix Stockham FFTs over lines held in LDS.  Each line plan carries a
 // compact twiddle table generated in FP64 on the host; a workgroup stages it
@@ -27,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "border1d.hpp"
 #include "device_common.hpp"
 #include "fft_bfly.hpp"
 #include "launch.hpp"
@@ -60,61 +62,77 @@ struct LinePlan {
     int post;               // offset of W_F^k, k = 0..L, for the real-even post/pre-processing; -1 if none
     const Cx<R>* tw;        // compact table in global memory
     int rad[kMaxStages];
-    int base[kMaxStages];   // stage twiddles W_{Ns*P}^(r*k) at base + k*(P-1) + r-1
+    int base[kMaxStages];   // stage twiddles W_{Ns*P}^(r*k) at base + (r-1)*Ns + k (consecutive k adjacent)
     int wp[kMaxStages];     // radix-P roots W_P^m at wp + m (stages without a fixed butterfly)
 };
 
+// LDS layout of a line: element t sits at pp(t), one spare slot after every
+// 16 elements.  The early Stockham stages write with stride P and column tiles
+// keep rows 16*Ct apart; without the padding those accesses fall into a few
+// banks (up to 16-way conflicts on the first radix-8 stage of a row).
+__host__ __device__ __forceinline__ int pp(int t) { return t + (t >> 4); }
+// LDS slots a line of L elements spans in that layout.
+__host__ __device__ __forceinline__ int pp_len(int L) { return L + (L >> 4) + 1; }
+
 // ---------------------------------------------------------------------------
-// Stockham stages over lines in LDS: element (line b, index t) at b*ls + t*es.
+// Stockham stages over lines in LDS: element (line b, index t) at b*ls + pp(t)*es.
 // LINE_FAST selects the thread->item order: line index fastest for column
 // tiles (es = #lines, consecutive lines adjacent in LDS), element index
 // fastest for rows (es = 1).  `tw` points at the plan's table in LDS.
 // ---------------------------------------------------------------------------
 
-template <typename R, int P, bool LINE_FAST>
+template <typename R, int P, bool LINE_FAST, bool INV>
 __device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
-                                            int nlines, int ls, int es, const Cx<R>* __restrict__ twb, bool inv) {
+                                            int nlines, int ls, int es, const Cx<R>* __restrict__ twb, int T) {
     using C = Cx<R>;
     const int nbf = L / P;
     const int total = nbf * nlines;
     const int dv = LINE_FAST ? nlines : nbf;
     const float rdv = 1.0f / float(dv);
     const float rns = 1.0f / float(Ns);
-    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+    for (int it = threadIdx.x; it < total; it += T) {
         int b, j;
         if (LINE_FAST) { j = fdiv(it, dv, rdv); b = it - j * dv; }
         else { b = fdiv(it, dv, rdv); j = it - b * dv; }
         const int q = fdiv(j, Ns, rns);
         const int k = j - q * Ns;
         const C* s = src + b * ls;
-        C v[P];
+        // All LDS reads of the butterfly (data and twiddles) are issued before
+        // any arithmetic: left to itself the scheduler interleaves the twiddle
+        // reads with the multiplies and serialises several LDS round trips.
+        C v[P], w[P];
 #pragma unroll
-        for (int r = 0; r < P; ++r) v[r] = s[(j + r * nbf) * es];
+        for (int r = 0; r < P; ++r) v[r] = s[pp(j + r * nbf) * es];
         if (Ns > 1) {
-            const C* w = twb + k * (P - 1) - 1;
+            const C* wb = twb + k - Ns;
 #pragma unroll
-            for (int r = 1; r < P; ++r) v[r] = cmulc(v[r], w[r], inv);
+            for (int r = 1; r < P; ++r) w[r] = wb[r * Ns];
         }
-        bfly<P>(v, inv);
+        __builtin_amdgcn_sched_barrier(0);
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < P; ++r) v[r] = cmulc(v[r], w[r], INV);
+        }
+        bfly<P>(v, INV);
         C* d = dst + b * ls;
         const int o = q * Ns * P + k;
 #pragma unroll
-        for (int r = 0; r < P; ++r) d[(o + r * Ns) * es] = v[r];
+        for (int r = 0; r < P; ++r) d[pp(o + r * Ns) * es] = v[r];
     }
 }
 
 // Any radix: one output per item, O(P) reads (used for the primes above 7
 // that good_size can produce: 11, 13, 17, 31, 43, 73, 127, 257).
-template <typename R, bool LINE_FAST>
+template <typename R, bool LINE_FAST, bool INV>
 __device__ __forceinline__ void stage_generic(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
                                               int P, int nlines, int ls, int es, const Cx<R>* __restrict__ twb,
-                                              const Cx<R>* __restrict__ wP, bool inv) {
+                                              const Cx<R>* __restrict__ wP, int T) {
     using C = Cx<R>;
     const int nbf = L / P;
     const int total = L * nlines;
     const int dv = LINE_FAST ? nlines : L;
     const float rdv = 1.0f / float(dv), rns = 1.0f / float(Ns), rp = 1.0f / float(P);
-    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+    for (int it = threadIdx.x; it < total; it += T) {
         int b, o;
         if (LINE_FAST) { o = fdiv(it, dv, rdv); b = it - o * dv; }
         else { b = fdiv(it, dv, rdv); o = it - b * dv; }
@@ -127,37 +145,43 @@ __device__ __forceinline__ void stage_generic(const Cx<R>* __restrict__ src, Cx<
         C acc = cmake<C>(R(0), R(0));
         int m = 0;   // (r * rout) mod P
         for (int r = 0; r < P; ++r) {
-            C v = s[(j + r * nbf) * es];
-            if (Ns > 1 && r) v = cmulc(v, twb[k * (P - 1) + r - 1], inv);
-            acc = cadd(acc, cmulc(v, wP[m], inv));
+            C v = s[pp(j + r * nbf) * es];
+            if (Ns > 1 && r) v = cmulc(v, twb[(r - 1) * Ns + k], INV);
+            acc = cadd(acc, cmulc(v, wP[m], INV));
             m += rout;
             if (m >= P) m -= P;
         }
-        dst[b * ls + o * es] = acc;
+        dst[b * ls + pp(o) * es] = acc;
     }
 }
 
 // Runs all stages; input in b0; returns the index (0/1) of the buffer holding
-// the output.  Ends with a barrier.
-template <typename R, bool LINE_FAST>
-__device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, const Cx<R>* tw, int nlines, int ls, int es,
-                         bool inv) {
-    Cx<R>* bufs[2] = {b0, b1};
+// the output.  Ends with a barrier.  The direction is a template parameter of
+// the stages so that no butterfly carries a runtime conjugation select.
+template <typename R, bool LINE_FAST, bool INV>
+__device__ __forceinline__ int fft_lines_dir(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, const Cx<R>* tw,
+                                             int nlines, int ls, int es, int T) {
+    // The two buffers are picked by selects, not through a local pointer
+    // array: an array of pointers hides that both live in LDS, and every
+    // access would then be compiled as a flat (generic) access.
     int cur = 0;
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int P = pl.rad[s];
-        const Cx<R>* src = bufs[cur];
-        Cx<R>* dst = bufs[cur ^ 1];
+        const Cx<R>* src = cur ? b1 : b0;
+        Cx<R>* dst = cur ? b0 : b1;
         const Cx<R>* twb = tw + pl.base[s];
         switch (P) {
-            case 2: stage_fixed<R, 2, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            case 3: stage_fixed<R, 3, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            case 4: stage_fixed<R, 4, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            case 5: stage_fixed<R, 5, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            case 7: stage_fixed<R, 7, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            case 8: stage_fixed<R, 8, LINE_FAST>(src, dst, pl.L, Ns, nlines, ls, es, twb, inv); break;
-            default: stage_generic<R, LINE_FAST>(src, dst, pl.L, Ns, P, nlines, ls, es, twb, tw + pl.wp[s], inv); break;
+            case 2: stage_fixed<R, 2, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 3: stage_fixed<R, 3, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 4: stage_fixed<R, 4, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 5: stage_fixed<R, 5, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 7: stage_fixed<R, 7, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 8: stage_fixed<R, 8, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            case 16: stage_fixed<R, 16, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+            default:
+                stage_generic<R, LINE_FAST, INV>(src, dst, pl.L, Ns, P, nlines, ls, es, twb, tw + pl.wp[s], T);
+                break;
         }
         __syncthreads();
         cur ^= 1;
@@ -166,10 +190,19 @@ __device__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, const Cx<R
     return cur;
 }
 
+// T is the block's thread count, read once at kernel entry: blockDim.x is a
+// load from the dispatch packet, and the barriers between stages would make
+// every stage issue it again and wait for it.
+template <typename R, bool LINE_FAST>
+__device__ __forceinline__ int fft_lines(Cx<R>* b0, Cx<R>* b1, const LinePlan<R>& pl, const Cx<R>* tw, int nlines,
+                                         int ls, int es, bool inv, int T) {
+    return inv ? fft_lines_dir<R, LINE_FAST, true>(b0, b1, pl, tw, nlines, ls, es, T)
+               : fft_lines_dir<R, LINE_FAST, false>(b0, b1, pl, tw, nlines, ls, es, T);
+}
+
 // Copies n entries global -> LDS with kU loads in flight per thread.
 template <typename C>
-__device__ __forceinline__ void stage_table(C* __restrict__ dst, const C* __restrict__ src, int n) {
-    const int T = blockDim.x;
+__device__ __forceinline__ void stage_table(C* __restrict__ dst, const C* __restrict__ src, int n, int T) {
     for (int base = threadIdx.x; base < n; base += kU * T) {
         C v[kU];
 #pragma unroll
@@ -185,20 +218,83 @@ __device__ __forceinline__ void stage_table(C* __restrict__ dst, const C* __rest
     }
 }
 
+// Twiddle staging split around a kernel's data loads: begin() issues the
+// global loads of the first kTabU * blockDim entries into registers, end()
+// writes them to LDS and copies any remainder, so the table's load latency
+// overlaps the data's instead of preceding it.
+template <typename C>
+struct TableStage {
+    static constexpr int kTabU = 4;
+    C v[kTabU];
+    __device__ __forceinline__ void begin(const C* __restrict__ src, int n, int T) {
+#pragma unroll
+        for (int u = 0; u < kTabU; ++u) {
+            const int i = threadIdx.x + u * T;
+            if (i < n) v[u] = src[i];
+        }
+    }
+    __device__ __forceinline__ void end(C* __restrict__ dst, const C* __restrict__ src, int n, int T) {
+#pragma unroll
+        for (int u = 0; u < kTabU; ++u) {
+            const int i = threadIdx.x + u * T;
+            if (i < n) dst[i] = v[u];
+        }
+        for (int i = threadIdx.x + kTabU * T; i < n; i += T) dst[i] = src[i];
+    }
+};
+
+// Copies a contiguous line of n entries global -> LDS in the padded layout.
+template <typename C>
+__device__ __forceinline__ void stage_line(C* __restrict__ dst, const C* __restrict__ src, int n, int T) {
+    for (int base = threadIdx.x; base < n; base += kU * T) {
+        C v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = base + u * T;
+            if (i < n) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = base + u * T;
+            if (i < n) dst[pp(i)] = v[u];
+        }
+    }
+}
+
+// Entries of a row plan's table that the row passes stage into LDS: the
+// stage twiddles; the real-even post/pre-processing factors W_F^k (used
+// once per bin) are read from global memory where they are needed.
+template <typename R>
+__host__ __device__ __forceinline__ int row_stage_tw(const LinePlan<R>& pl) {
+    return pl.post >= 0 ? pl.post : pl.ntw;
+}
+
 // ---------------------------------------------------------------------------
 // Load stages
 // ---------------------------------------------------------------------------
 
 // What the data side needs to read the padded input through the border maps.
 template <typename E>
+// The per-axis border map is evaluated in closed form (resolve1d), so no
+// load phase waits on a map lookup.
 struct PadSrc {
     const E* x;
-    const int32_t* map[kMaxRank];   // length fft[e]
     E cst[kMaxRank][2];
     int64_t xstr[kMaxRank];
     int32_t lo[kMaxRank];           // interior [lo, lo + n) maps to x index p - lo
     int32_t n[kMaxRank];
+    int32_t pds[kMaxRank];          // padded extent; cells beyond it are zero
+    int32_t side[kMaxRank][2];      // Side1D of the front and back fill
 };
+
+// Border-map code of padded cell q of axis e (plan.cpp axis_map, position by
+// position): an index into x, or kUnwritten / kConstFront / kConstBack /
+// kOutside.
+template <typename E>
+__device__ __forceinline__ int32_t axis_code(const PadSrc<E>& s, int e, int q) {
+    if (q >= s.pds[e]) return kOutside;
+    return resolve1d<int>(q, s.n[e], s.lo[e], s.side[e][0], s.side[e][1]);
+}
 
 // What the kernel side needs to read the scattered, dilated kernel.
 template <typename E>
@@ -235,15 +331,15 @@ __device__ __forceinline__ E zero_of() {
 // Value of the padded row at position p of the last axis.  Interior
 // positions are computed without touching the border map.
 template <typename E>
-__device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, int ax, int64_t p) {
+__device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, int ax, int p) {
     if (rr.zero) return zero_of<E>();
-    const int64_t rel = p - s.lo[ax];
+    const int rel = p - s.lo[ax];
     if (rel >= 0 && rel < s.n[ax]) {
         if (rr.oc_axis >= 0) return rr.ocval;
         if (rr.unw) return zero_of<E>();
-        return s.x[rr.xoff + rel * s.xstr[ax]];
+        return s.x[rr.xoff + int64_t(rel) * s.xstr[ax]];
     }
-    const int32_t c = s.map[ax][p];
+    const int32_t c = axis_code(s, ax, p);
     if (c == kOutside) return zero_of<E>();
     if (c <= kConstFront) return s.cst[ax][c == kConstFront ? 0 : 1];
     if (rr.oc_axis >= 0) return rr.ocval;
@@ -251,13 +347,91 @@ __device__ __forceinline__ E pad_value(const PadSrc<E>& s, const RowRes<E>& rr, 
     return s.x[rr.xoff + int64_t(c) * s.xstr[ax]];
 }
 
+// Kernel tap at position p of the scattered, dilated line (32-bit: p is an
+// index into an LDS-resident line).
 template <typename E>
-__device__ __forceinline__ E ker_value(const KerSrc<E>& s, int64_t koff, int ax, int64_t p) {
+__device__ __forceinline__ E ker_value(const KerSrc<E>& s, int64_t koff, int ax, int p) {
     const int dd = s.d[ax];
-    if (p % dd) return zero_of<E>();
-    const int64_t i = p / dd;
-    if (i >= s.K[ax]) return zero_of<E>();
-    return s.k[koff + (s.rev ? i : s.K[ax] - 1 - i) * s.kstr[ax]];
+    const int i = dd == 1 ? p : int(unsigned(p) / unsigned(dd));
+    if (i * dd != p || i >= s.K[ax]) return zero_of<E>();
+    return s.k[koff + int64_t(s.rev ? i : s.K[ax] - 1 - i) * s.kstr[ax]];
+}
+
+// Last-axis constants of the data border map, read once per kernel.
+template <typename E>
+struct LastAxisPad {
+    int lo, n, pds, front, back;
+    int64_t xs;
+    E cfront, cback;
+};
+
+template <typename E>
+__device__ __forceinline__ LastAxisPad<E> last_axis_pad(const PadSrc<E>& s, int ax) {
+    LastAxisPad<E> la;
+    la.lo = s.lo[ax];
+    la.n = s.n[ax];
+    la.pds = s.pds[ax];
+    la.front = s.side[ax][0];
+    la.back = s.side[ax][1];
+    la.xs = s.xstr[ax];
+    la.cfront = s.cst[ax][0];
+    la.cback = s.cst[ax][1];
+    return la;
+}
+
+// Whether padded cell p of a data row reads x: if so *off is its offset into
+// x (negative for views with negative strides), else *fill holds the cell's
+// value.  pad_value's rule with no memory access, so that a caller can issue
+// all of a round's loads before using any of them.
+template <typename E>
+__device__ __forceinline__ bool pad_offset(const LastAxisPad<E>& la, const RowRes<E>& rr, int p, int64_t* off,
+                                           E* fill) {
+    *fill = zero_of<E>();
+    *off = 0;
+    if (rr.zero) return false;
+    const int rel = p - la.lo;
+    int c;
+    if (rel >= 0 && rel < la.n) c = rel;
+    else if (p >= la.pds) c = kOutside;
+    else c = resolve1d<int>(p, la.n, la.lo, la.front, la.back);
+    if (c == kOutside) return false;
+    if (c == kConstFront || c == kConstBack) {
+        *fill = c == kConstFront ? la.cfront : la.cback;
+        return false;
+    }
+    if (rr.oc_axis >= 0) {
+        *fill = rr.ocval;
+        return false;
+    }
+    if (c == kUnwritten || rr.unw) return false;
+    *off = rr.xoff + int64_t(c) * la.xs;
+    return true;
+}
+
+// Last-axis constants of the kernel scatter.
+struct LastAxisKer {
+    int d, K, rev;
+    int64_t ks;
+};
+
+template <typename E>
+__device__ __forceinline__ LastAxisKer last_axis_ker(const KerSrc<E>& s, int ax) {
+    LastAxisKer k;
+    k.d = s.d[ax];
+    k.K = s.K[ax];
+    k.rev = s.rev;
+    k.ks = s.kstr[ax];
+    return k;
+}
+
+// Whether position p of a scattered kernel row holds a tap; if so *off is
+// its offset into k (negative for views with negative strides).
+__device__ __forceinline__ bool ker_offset(const LastAxisKer& k, int64_t koff, int p, int64_t* off) {
+    const int i = k.d == 1 ? p : int(unsigned(p) / unsigned(k.d));
+    *off = 0;
+    if (i * k.d != p || i >= k.K) return false;
+    *off = koff + int64_t(k.rev ? i : k.K - 1 - i) * k.ks;
+    return true;
 }
 
 template <typename R, typename E>
@@ -277,16 +451,28 @@ __device__ __forceinline__ Cx<R> row_sample(int mode, int n, F&& val) {
     return to_cx<R>(val(n));
 }
 
-// X[k] of a real line of length F = 2M from the half-length spectrum Z
+// X[k] of a real line of length F = 2M from the half-length spectrum Z (an
+// LDS line in the padded layout)
 // (X = E + W_F^k O with E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2).
 template <typename C>
 __device__ __forceinline__ C r2c_bin(const C* Z, int M, int k, const C* wpost) {
     using R = decltype(C{}.x);
-    const C zk = Z[k == M ? 0 : k];
-    const C zm = cconj(Z[k == 0 ? 0 : M - k]);
+    const C zk = Z[pp(k == M ? 0 : k)];
+    const C zm = cconj(Z[pp(k == 0 ? 0 : M - k)]);
     const C e = cscale(cadd(zk, zm), R(0.5));
     const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
     return cadd(e, cmul(o, wpost[k]));
+}
+
+// r2c_bin with the factor W_F^k passed by value.
+template <typename C>
+__device__ __forceinline__ C r2c_bin_w(const C* Z, int M, int k, C w) {
+    using R = decltype(C{}.x);
+    const C zk = Z[pp(k == M ? 0 : k)];
+    const C zm = cconj(Z[pp(k == 0 ? 0 : M - k)]);
+    const C e = cscale(cadd(zk, zm), R(0.5));
+    const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
+    return cadd(e, cmul(o, w));
 }
 
 // Z[k] of the half-length inverse from bins A = X[k] and B = conj X[M-k]:
@@ -324,9 +510,14 @@ struct RowsFwdArgs {
 template <typename R, typename E>
 __global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E> a) {
     using C = Cx<R>;
+    constexpr bool kCE = std::is_same<E, C>::value;
     extern __shared__ unsigned char smem_raw[];
+    const int T = blockDim.x;
+    const int ntw = row_stage_tw(a.pl);
+    TableStage<C> tab;
+    tab.begin(a.pl.tw, ntw, T);
     C* tw = reinterpret_cast<C*>(smem_raw);
-    C* b0 = tw + a.pl.ntw;
+    C* b0 = tw + ntw;
     C* b1 = b0 + a.lpb * a.ls;
     __shared__ int64_t s_off[kMaxLpb];
     __shared__ int64_t s_xoff[kMaxLpb];
@@ -334,34 +525,34 @@ __global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E>
     __shared__ E s_oc[kMaxLpb];
 
     const int job = blockIdx.x < unsigned(a.nblocks0) ? 0 : 1;
-    const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
-    const int64_t line0 = blk * a.lpb;
+    const int blk = job == 0 ? int(blockIdx.x) : int(blockIdx.x) - a.nblocks0;
+    const int line0 = blk * a.lpb;
     const int nl = int(imin64(a.lpb, a.nlines[job] - line0));
     const int ax = a.nout;   // last axis index
 
-    // per-line outer resolution
+    // per-line outer resolution (closed-form maps: no memory reads)
     if (threadIdx.x < nl) {
-        int64_t rem = line0 + threadIdx.x;
+        int rem = line0 + int(threadIdx.x);
         int64_t soff = 0, xoff = 0;
         int flag = 0;
         int oc = -1;
         E ocv = zero_of<E>();
         for (int e = a.nout - 1; e >= 0; --e) {
-            const int64_t q = rem % a.cnt[job][e];
-            rem /= a.cnt[job][e];
-            soff += q * a.sstr[e];
+            const int cnt = int(a.cnt[job][e]);
+            const int q = rem % cnt;
+            rem /= cnt;
+            soff += int64_t(q) * a.sstr[e];
             if (job == 0) {
-                const int32_t c = a.pad.map[e][q];
+                const int32_t c = axis_code(a.pad, e, q);
                 if (c >= 0) xoff += int64_t(c) * a.pad.xstr[e];
                 else if (c == kOutside) flag |= 1;
                 else if (c == kUnwritten) flag |= 2;
                 else if (oc < e) { oc = e; ocv = a.pad.cst[e][c == kConstFront ? 0 : 1]; }
             } else {
-                if (q % a.ker.d[e] != 0) flag |= 1;
-                const int64_t i = q / a.ker.d[e];
-                if (i >= a.ker.K[e]) flag |= 1;
-                const int64_t ki = a.ker.rev ? i : a.ker.K[e] - 1 - i;
-                xoff += ki * a.ker.kstr[e];
+                const int i = q / a.ker.d[e];
+                if (i * a.ker.d[e] != q || i >= a.ker.K[e]) flag |= 1;
+                const int ki = a.ker.rev ? i : a.ker.K[e] - 1 - i;
+                xoff += int64_t(ki) * a.ker.kstr[e];
             }
         }
         s_off[threadIdx.x] = soff;
@@ -369,55 +560,95 @@ __global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E>
         s_flag[threadIdx.x] = flag | ((oc + 1) << 8);
         s_oc[threadIdx.x] = ocv;
     }
-    stage_table(tw, a.pl.tw, a.pl.ntw);
     __syncthreads();
 
-    // batched load of the rows
+    // Batched load of the rows.  Every sample's source offset (or fill value)
+    // is computed without touching memory, then all loads of the round are
+    // issued back to back from clamped addresses and the fills selected
+    // afterwards: no load sits behind a branch, so they overlap.
     const int L = a.pl.L;
     const int total = nl * L;
-    const int T = blockDim.x;
     const float rL = 1.0f / float(L);
+    const LastAxisPad<E> la = last_axis_pad(a.pad, ax);
+    const LastAxisKer lk = last_axis_ker(a.ker, ax);
+    const E* src = job == 0 ? a.pad.x : a.ker.k;
+    constexpr int kS = kCE ? 1 : 2;               // samples per complex point (half-length real rows)
+    const int spt = (!kCE && a.mode == 0) ? 2 : 1;
     for (int base = threadIdx.x; base < total; base += kU * T) {
-        C z[kU];
+        int64_t off[kU][kS];
+        bool ok[kU][kS];
+        E fill[kU][kS];
+        int at[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int it = base + u * T;
-            z[u] = cmake<C>(R(0), R(0));
-            if (it < total) {
-                const int b = fdiv(it, L, rL), n = it - b * L;
-                const int flag = s_flag[b];
-                if (!(flag & 1)) {
-                    if (job == 0) {
-                        const RowRes<E> rr{s_xoff[b], (flag >> 8) - 1, s_oc[b], (flag & 2) != 0, false};
-                        z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return pad_value(a.pad, rr, ax, p); });
-                    } else {
-                        const int64_t koff = s_xoff[b];
-                        z[u] = row_sample<R, E>(a.mode, n, [&](int p) { return ker_value(a.ker, koff, ax, p); });
-                    }
+            const int itc = it < total ? it : 0;
+            const int b = fdiv(itc, L, rL), n = itc - b * L;
+            at[u] = b * a.ls + pp(n);
+            const int flag = s_flag[b];
+            const RowRes<E> rr{s_xoff[b], (flag >> 8) - 1, s_oc[b], (flag & 2) != 0, (flag & 1) != 0 || it >= total};
+#pragma unroll
+            for (int q = 0; q < kS; ++q) {
+                fill[u][q] = zero_of<E>();
+                off[u][q] = 0;
+                ok[u][q] = false;
+                if (q < spt) {
+                    const int pos = spt * n + q;
+                    if (job == 0) ok[u][q] = pad_offset(la, rr, pos, &off[u][q], &fill[u][q]);
+                    else if (!rr.zero) ok[u][q] = ker_offset(lk, rr.xoff, pos, &off[u][q]);
                 }
             }
         }
+        E v[kU][kS];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int q = 0; q < kS; ++q)
+                if (q < spt) v[u][q] = src[off[u][q]];   // offset 0 (a valid element) where !ok
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int it = base + u * T;
-            if (it < total) {
-                const int b = fdiv(it, L, rL), n = it - b * L;
-                b0[b * a.ls + n] = z[u];
+            if (base + u * T >= total) continue;
+            if constexpr (kCE) {
+                b0[at[u]] = ok[u][0] ? v[u][0] : fill[u][0];
+            } else {
+                const E v0 = ok[u][0] ? v[u][0] : fill[u][0];
+                const E v1 = spt == 2 ? (ok[u][1] ? v[u][1] : fill[u][1]) : E(0);
+                b0[at[u]] = cmake<C>(R(v0), R(v1));
             }
         }
     }
+    tab.end(tw, a.pl.tw, ntw, T);
     __syncthreads();
-    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, false);
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, false, T);
     const C* Z = cur ? b1 : b0;
 
-    // store H bins per row
+    // store H bins per row; the post-processing factors of a round are loaded
+    // together before any of them is used
     Cx<R>* out = a.spec[job];
     const int H = a.H;
+    const int nH = nl * H;
     const float rH = 1.0f / float(H);
-    for (int it = threadIdx.x; it < nl * H; it += T) {
-        const int b = fdiv(it, H, rH), k = it - b * H;
-        const C X = a.mode == 0 ? r2c_bin(Z + b * a.ls, L, k, tw + a.pl.post) : Z[b * a.ls + k];
-        out[s_off[b] + k] = X;
+    const C* wpost = a.pl.tw + a.pl.post;
+    for (int base = threadIdx.x; base < nH; base += kU * T) {
+        C wk[kU];
+        if (a.mode == 0) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int it = base + u * T;
+                const int itc = it < nH ? it : 0;
+                const int b = fdiv(itc, H, rH), k = itc - b * H;
+                wk[u] = wpost[k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = base + u * T;
+            if (it >= nH) continue;
+            const int b = fdiv(it, H, rH), k = it - b * H;
+            const C* Zb = Z + b * a.ls;
+            const C X = a.mode == 0 ? r2c_bin_w(Zb, L, k, wk[u]) : Zb[pp(k)];
+            out[s_off[b] + k] = X;
+        }
     }
 }
 
@@ -455,10 +686,11 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     extern __shared__ unsigned char smem_raw[];
     const int L = a.pl.L;
     const int Ct = a.C;
+    const int PL = pp_len(L) * Ct;   // LDS slots of one tile buffer
     C* tw = reinterpret_cast<C*>(smem_raw);
     C* b0 = tw + a.pl.ntw;
-    C* b1 = b0 + L * Ct;
-    C* b2 = b1 + L * Ct;
+    C* b1 = b0 + PL;
+    C* b2 = b1 + PL;
     const int T = blockDim.x;
 
     const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
@@ -481,59 +713,59 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         const int total = L * Ct;
         for (int b = threadIdx.x; b < total; b += kU * T) {
             C v[kU];
+            int at[kU];
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int it = b + u * T;
                 v[u] = cmake<C>(R(0), R(0));
-                if (it < total) {
-                    const int t = fdiv(it, Ct, rC), c = it - t * Ct;
-                    if (t < valid && c < nc) v[u] = src[base + int64_t(t) * sd + c0 + c];
-                }
+                const int t = fdiv(it, Ct, rC), c = it - t * Ct;
+                at[u] = pp(t) * Ct + c;
+                if (it < total && t < valid && c < nc) v[u] = src[base + int64_t(t) * sd + c0 + c];
             }
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int it = b + u * T;
-                if (it < total) dstbuf[it] = v[u];
+                if (it < total) dstbuf[at[u]] = v[u];
             }
         }
     };
 
-    stage_table(tw, a.pl.tw, a.pl.ntw);
+    stage_table(tw, a.pl.tw, a.pl.ntw, T);
     C* res;
     if (a.mode == 0) {
         load(a.spec[job], b0, a.valid[job]);
         __syncthreads();
-        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, false) ? b1 : b0;
+        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, false, T) ? b1 : b0;
     } else if (a.mode == 1) {
         // kernel and data columns loaded together, then transformed in turn
         load(a.spec[1], b1, a.valid[1]);
         load(a.spec[0], b0, a.valid[0]);
         __syncthreads();
-        const int ck = fft_lines<R, true>(b1, b2, a.pl, tw, Ct, 1, Ct, false);
+        const int ck = fft_lines<R, true>(b1, b2, a.pl, tw, Ct, 1, Ct, false, T);
         C* kres = ck ? b2 : b1;
         C* kfree = ck ? b1 : b2;
-        const int cd = fft_lines<R, true>(b0, kfree, a.pl, tw, Ct, 1, Ct, false);
+        const int cd = fft_lines<R, true>(b0, kfree, a.pl, tw, Ct, 1, Ct, false, T);
         C* dres = cd ? kfree : b0;
         C* dfree = cd ? b0 : kfree;
-        for (int it = threadIdx.x; it < L * Ct; it += T) dres[it] = cmul(dres[it], kres[it]);
+        for (int it = threadIdx.x; it < PL; it += T) dres[it] = cmul(dres[it], kres[it]);
         __syncthreads();
-        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true) ? dfree : dres;
+        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true, T) ? dfree : dres;
     } else if (a.mode == 3) {
         // kernel columns already transformed (kept spectrum): data forward,
         // product, inverse
         load(a.spec[1], b1, L);
         load(a.spec[0], b0, a.valid[0]);
         __syncthreads();
-        const int cd = fft_lines<R, true>(b0, b2, a.pl, tw, Ct, 1, Ct, false);
+        const int cd = fft_lines<R, true>(b0, b2, a.pl, tw, Ct, 1, Ct, false, T);
         C* dres = cd ? b2 : b0;
         C* dfree = cd ? b0 : b2;
-        for (int it = threadIdx.x; it < L * Ct; it += T) dres[it] = cmul(dres[it], b1[it]);
+        for (int it = threadIdx.x; it < PL; it += T) dres[it] = cmul(dres[it], b1[it]);
         __syncthreads();
-        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true) ? dfree : dres;
+        res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true, T) ? dfree : dres;
     } else {
         load(a.spec[0], b0, L);
         __syncthreads();
-        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, true) ? b1 : b0;
+        res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, true, T) ? b1 : b0;
     }
     // store selected rows
     C* dst = a.spec[job];
@@ -542,7 +774,7 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         const int i = fdiv(it, Ct, rC), c = it - i * Ct;
         if (c >= nc) continue;
         const int t = int(a.outsel.start + int64_t(i) * a.outsel.step);
-        dst[base + int64_t(t) * sd + c0 + c] = res[t * Ct + c];
+        dst[base + int64_t(t) * sd + c0 + c] = res[pp(t) * Ct + c];
     }
 }
 
@@ -568,13 +800,13 @@ struct RowsInvArgs {
 template <typename R, typename E>
 __device__ __forceinline__ E store_value(int mode, const Cx<R>* Z, int64_t p, R scale) {
     if constexpr (std::is_same<E, Cx<R>>::value) {
-        return cscale(Z[p], scale);
+        return cscale(Z[pp(int(p))], scale);
     } else {
         if (mode == 0) {
-            const Cx<R> zz = Z[p >> 1];
+            const Cx<R> zz = Z[pp(int(p >> 1))];
             return E(((p & 1) ? zz.y : zz.x) * scale);
         }
-        return E(Z[p].x * scale);
+        return E(Z[pp(int(p))].x * scale);
     }
 }
 
@@ -582,14 +814,17 @@ template <typename R, typename E>
 __global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E> a) {
     using C = Cx<R>;
     extern __shared__ unsigned char smem_raw[];
+    const int ntw = row_stage_tw(a.pl);
     C* tw = reinterpret_cast<C*>(smem_raw);
-    C* b0 = tw + a.pl.ntw;
+    C* b0 = tw + ntw;
     C* b1 = b0 + a.lpb * a.ls;
     __shared__ int64_t s_off[kMaxLpb];
     __shared__ int64_t s_yoff[kMaxLpb];
     const int T = blockDim.x;
     const int64_t line0 = int64_t(blockIdx.x) * a.lpb;
     const int nl = int(imin64(a.lpb, a.nlines - line0));
+    TableStage<C> tab;
+    tab.begin(a.pl.tw, ntw, T);
     if (threadIdx.x < nl) {
         int64_t rem = line0 + threadIdx.x;
         int64_t soff = 0, yoff = 0;
@@ -602,10 +837,10 @@ __global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E>
         s_off[threadIdx.x] = soff;
         s_yoff[threadIdx.x] = yoff;
     }
-    stage_table(tw, a.pl.tw, a.pl.ntw);
     __syncthreads();
     const int L = a.pl.L;
     const int M = L;
+    const C* wpost = a.pl.tw + a.pl.post;
     const float rL = 1.0f / float(L);
     const int total = nl * L;
     for (int base = threadIdx.x; base < total; base += kU * T) {
@@ -621,7 +856,7 @@ __global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E>
                     // Im X[0] and Im X[M] are ignored as a c2r transform does
                     C A = X[k], B = cconj(X[M - k]);
                     if (k == 0) { A.y = 0; B.y = 0; }
-                    z[u] = c2r_pre(A, B, k, tw + a.pl.post);
+                    z[u] = c2r_pre(A, B, k, wpost);
                 } else if (a.mode == 1) {
                     // odd F: Hermitian completion of the half spectrum
                     if (k == 0) z[u] = cmake<C>(X[0].x, R(0));
@@ -637,12 +872,13 @@ __global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E>
             const int it = base + u * T;
             if (it < total) {
                 const int b = fdiv(it, L, rL), k = it - b * L;
-                b0[b * a.ls + k] = z[u];
+                b0[b * a.ls + pp(k)] = z[u];
             }
         }
     }
+    tab.end(tw, a.pl.tw, ntw, T);
     __syncthreads();
-    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, true);
+    const int cur = fft_lines<R, false>(b0, b1, a.pl, tw, nl, a.ls, 1, true, T);
     const C* Z = cur ? b1 : b0;
     const int nout_last = int(a.lastsel.count);
     const float rn = 1.0f / float(nout_last);
@@ -677,7 +913,9 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
     extern __shared__ unsigned char smem_raw[];
     const int L = a.pl.L;
     C* tw = reinterpret_cast<C*>(smem_raw);
-    C* buf[3] = {tw + a.pl.ntw, tw + a.pl.ntw + L, tw + a.pl.ntw + 2 * L};
+    C* const buf0 = tw + a.pl.ntw;
+    C* const buf1 = buf0 + pp_len(L);
+    C* const buf2 = buf1 + pp_len(L);
     const int T = blockDim.x;
     const RowRes<E> rr{0, -1, zero_of<E>(), false, false};
     auto load = [&](C* dst, bool kernel) {
@@ -695,26 +933,26 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int n = base + u * T;
-                if (n < L) dst[n] = z[u];
+                if (n < L) dst[pp(n)] = z[u];
             }
         }
     };
-    stage_table(tw, a.pl.tw, a.pl.ntw);
+    stage_table(tw, a.pl.tw, a.pl.ntw, T);
     C* K = nullptr;
-    C* f0 = buf[0];
-    C* f1 = buf[1];
+    C* f0 = buf0;
+    C* f1 = buf1;
     if (!a.ks) {
-        load(buf[2], true);
-        load(buf[0], false);
+        load(buf2, true);
+        load(buf0, false);
         __syncthreads();
-        const int ck = fft_lines<R, false>(buf[2], buf[1], a.pl, tw, 1, L, 1, false);
-        K = ck ? buf[1] : buf[2];
-        f1 = ck ? buf[2] : buf[1];
+        const int ck = fft_lines<R, false>(buf2, buf1, a.pl, tw, 1, pp_len(L), 1, false, T);
+        K = ck ? buf1 : buf2;
+        f1 = ck ? buf2 : buf1;
     } else {
-        load(buf[0], false);
+        load(buf0, false);
         __syncthreads();
     }
-    const int cd = fft_lines<R, false>(f0, f1, a.pl, tw, 1, L, 1, false);
+    const int cd = fft_lines<R, false>(f0, f1, a.pl, tw, 1, pp_len(L), 1, false, T);
     C* D = cd ? f1 : f0;
     C* W = cd ? f0 : f1;   // free buffer
     const C* wpost = tw + a.pl.post;
@@ -727,23 +965,29 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
             C Pk = cmul(r2c_bin(D, M, k, wpost), a.ks ? a.ks[k] : r2c_bin(K, M, k, wpost));
             C Pm = cmul(r2c_bin(D, M, km, wpost), a.ks ? a.ks[km] : r2c_bin(K, M, km, wpost));
             if (k == 0) { Pk.y = 0; Pm.y = 0; }   // Im of DC and Nyquist ignored by c2r
-            W[k == M ? 0 : k] = c2r_pre(Pk, cconj(Pm), k, wpost);
-            if (km != k && km < M) W[km] = c2r_pre(Pm, cconj(Pk), km, wpost);
+            W[pp(k == M ? 0 : k)] = c2r_pre(Pk, cconj(Pm), k, wpost);
+            if (km != k && km < M) W[pp(km)] = c2r_pre(Pm, cconj(Pk), km, wpost);
         }
     } else if (a.mode == 1) {
         // odd F: the product of two Hermitian spectra, used whole
-        for (int k = threadIdx.x; k < L; k += T) {
+        if (a.ks) {
             // the stored kernel spectrum holds bins 0..F/2; the rest by symmetry
-            const C kk = a.ks ? (k < a.H ? a.ks[k] : cconj(a.ks[L - k])) : K[k];
-            W[k] = cmul(D[k], kk);
+            for (int k = threadIdx.x; k < L; k += T)
+                W[pp(k)] = cmul(D[pp(k)], k < a.H ? a.ks[k] : cconj(a.ks[L - k]));
+        } else {
+            for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], K[pp(k)]);
         }
         if (threadIdx.x == 0) W[0].y = 0;
+    } else if (a.ks) {
+        // separate loops: a select between the global and the LDS pointer
+        // would turn both into flat accesses
+        for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], a.ks[k]);
     } else {
-        for (int k = threadIdx.x; k < L; k += T) W[k] = cmul(D[k], a.ks ? a.ks[k] : K[k]);
+        for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], K[pp(k)]);
     }
     __syncthreads();
     C* other = (W == f0) ? f1 : f0;
-    const int ci = fft_lines<R, false>(W, other, a.pl, tw, 1, L, 1, true);
+    const int ci = fft_lines<R, false>(W, other, a.pl, tw, 1, pp_len(L), 1, true, T);
     const C* Z = ci ? other : W;
     const int nout = int(a.lastsel.count);
     for (int i = threadIdx.x; i < nout; i += T) {
@@ -790,7 +1034,7 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
     const int L1 = a.pl.L, L2 = a.L2, Ct = a.Ct;
     C* tw = reinterpret_cast<C*>(smem_raw);
     C* b0 = tw + a.pl.ntw;
-    C* b1 = b0 + L1 * Ct;
+    C* b1 = b0 + pp_len(L1) * Ct;
     const int T = blockDim.x;
     const int job = (!a.inv && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
     const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
@@ -812,7 +1056,7 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
             yoff += i * a.ystr[e];
             if (a.inv) continue;
             if (job == 0) {
-                const int32_t c = a.pad.map[e][q];
+                const int32_t c = axis_code(a.pad, e, int(q));
                 if (c >= 0) xoff += int64_t(c) * a.pad.xstr[e];
                 else if (c == kOutside) zero = true;
                 else if (c == kUnwritten) unw = true;
@@ -829,17 +1073,20 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
     const int nc = min(Ct, L2 - c0);
     const int ax = a.nout;
     const float rC = 1.0f / float(Ct);
-    stage_table(tw, a.pl.tw, a.pl.ntw);
+    stage_table(tw, a.pl.tw, a.pl.ntw, T);
     const RowRes<E> rr{xoff, oc, ocv, unw, zero};
     const int total = L1 * Ct;
     for (int base = threadIdx.x; base < total; base += kU * T) {
         C z[kU];
+        int at[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int it = base + u * T;
             z[u] = cmake<C>(R(0), R(0));
+            at[u] = 0;
             if (it < total) {
                 const int n1 = fdiv(it, Ct, rC), c = it - n1 * Ct;
+                at[u] = pp(n1) * Ct + c;
                 if (c < nc) {
                     const int n2 = c0 + c;
                     if (!a.inv) {
@@ -855,11 +1102,11 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int it = base + u * T;
-            if (it < total) b0[it] = z[u];
+            if (it < total) b0[at[u]] = z[u];
         }
     }
     __syncthreads();
-    const int cur = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, a.inv != 0);
+    const int cur = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, a.inv != 0, T);
     const C* Z = cur ? b1 : b0;
     if (!a.inv) {
         C* out = a.spec[job];
@@ -867,7 +1114,7 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
             const int k1 = fdiv(it, Ct, rC), c = it - k1 * Ct;
             if (c >= nc) continue;
             const int n2 = c0 + c;
-            out[soff + int64_t(k1) * L2 + n2] = cmul(Z[it], a.twL[int64_t(k1) * n2]);
+            out[soff + int64_t(k1) * L2 + n2] = cmul(Z[pp(k1) * Ct + c], a.twL[int64_t(k1) * n2]);
         }
     } else {
         for (int it = threadIdx.x; it < total; it += T) {
@@ -879,8 +1126,8 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
             const int64_t i = rel / a.lastsel.step;
             if (i >= a.lastsel.count) continue;
             E v;
-            if constexpr (std::is_same<E, C>::value) v = cscale(Z[it], a.scale);
-            else v = E(Z[it].x * a.scale);
+            if constexpr (std::is_same<E, C>::value) v = cscale(Z[pp(n1) * Ct + c], a.scale);
+            else v = E(Z[pp(n1) * Ct + c].x * a.scale);
             a.y[yoff + i] = v;
         }
     }
@@ -904,8 +1151,8 @@ __global__ void __launch_bounds__(kMaxThreads) bigrow_kernel(BigRowArgs<R> a) {
     const int L2 = a.L2;
     C* tw = reinterpret_cast<C*>(smem_raw);
     C* b0 = tw + a.pl.ntw;
-    C* b1 = b0 + L2;
-    C* b2 = b1 + L2;
+    C* b1 = b0 + pp_len(L2);
+    C* b2 = b1 + pp_len(L2);
     const int T = blockDim.x;
     const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
     const int64_t row = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
@@ -918,39 +1165,48 @@ __global__ void __launch_bounds__(kMaxThreads) bigrow_kernel(BigRowArgs<R> a) {
         rem /= s.count;
         soff += (s.start + i * s.step) * a.sstr[e];
     }
-    stage_table(tw, a.pl.tw, a.pl.ntw);
+    stage_table(tw, a.pl.tw, a.pl.ntw, T);
     C* res;
     if (a.mode == 1) {
-        stage_table(b1, a.spec[1] + soff, L2);
-        stage_table(b0, a.spec[0] + soff, L2);
+        stage_line(b1, a.spec[1] + soff, L2, T);
+        stage_line(b0, a.spec[0] + soff, L2, T);
         __syncthreads();
-        const int ck = fft_lines<R, false>(b1, b2, a.pl, tw, 1, L2, 1, false);
+        const int ck = fft_lines<R, false>(b1, b2, a.pl, tw, 1, pp_len(L2), 1, false, T);
         C* K = ck ? b2 : b1;
         C* f1 = ck ? b1 : b2;
-        const int cd = fft_lines<R, false>(b0, f1, a.pl, tw, 1, L2, 1, false);
+        const int cd = fft_lines<R, false>(b0, f1, a.pl, tw, 1, pp_len(L2), 1, false, T);
         C* D = cd ? f1 : b0;
         C* Dfree = cd ? b0 : f1;
-        for (int n = threadIdx.x; n < L2; n += T) D[n] = cmul(D[n], K[n]);
+        for (int n = threadIdx.x; n < pp_len(L2); n += T) D[n] = cmul(D[n], K[n]);
         __syncthreads();
-        res = fft_lines<R, false>(D, Dfree, a.pl, tw, 1, L2, 1, true) ? Dfree : D;
+        res = fft_lines<R, false>(D, Dfree, a.pl, tw, 1, pp_len(L2), 1, true, T) ? Dfree : D;
     } else {
-        stage_table(b0, a.spec[job] + soff, L2);
+        stage_line(b0, a.spec[job] + soff, L2, T);
         __syncthreads();
-        res = fft_lines<R, false>(b0, b1, a.pl, tw, 1, L2, 1, a.mode == 2) ? b1 : b0;
+        res = fft_lines<R, false>(b0, b1, a.pl, tw, 1, pp_len(L2), 1, a.mode == 2, T) ? b1 : b0;
     }
-    for (int n = threadIdx.x; n < L2; n += T) a.spec[job][soff + n] = res[n];
+    for (int n = threadIdx.x; n < L2; n += T) a.spec[job][soff + n] = res[pp(n)];
 }
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
+// Radices of a line plan.  The power-of-two part takes the fewest stages of
+// radix at most 16, with the bits spread evenly over them (2^8 = 16*16,
+// 2^9 = 8*8*8, 2^10 = 16*8*8): fewer stages mean fewer LDS round trips and
+// barriers, and radix 8-16 butterflies cost the fewest instructions per point.
 std::vector<int> factor_radices(int L) {
     std::vector<int> r;
-    int n = L;
-    while (n % 8 == 0) { r.push_back(8); n /= 8; }
-    while (n % 4 == 0) { r.push_back(4); n /= 4; }
-    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    int n = L, a = 0;
+    while (n % 2 == 0) { n /= 2; ++a; }
+    if (a > 0) {
+        const int st = (a + 3) / 4;
+        for (int i = 0; i < st; ++i) {
+            const int bits = a / st + (i < a % st ? 1 : 0);
+            r.push_back(1 << bits);
+        }
+    }
     for (int p : {3, 5, 7}) while (n % p == 0) { r.push_back(p); n /= p; }
     for (int p = 11; n > 1; p += 2) {
         while (n % p == 0) { r.push_back(p); n /= p; }
@@ -959,7 +1215,7 @@ std::vector<int> factor_radices(int L) {
     return r;
 }
 
-bool fixed_radix(int p) { return p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8; }
+bool fixed_radix(int p) { return p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16; }
 
 // Entries of the compact twiddle table make_lineplan builds for (L, post_F).
 int64_t ntw_of(int L, int post_F) {
@@ -999,8 +1255,8 @@ bool make_lineplan(LinePlan<R>* pl, int L, int post_F, LaunchContext& ctx) {
         const int P = r[s];
         pl->rad[s] = P;
         pl->base[s] = int(tab.size() / 2);
-        for (int k = 0; k < Ns; ++k)
-            for (int q = 1; q < P; ++q) push(-2.0 * M_PI * double(int64_t(q) * k) / double(int64_t(Ns) * P));
+        for (int q = 1; q < P; ++q)
+            for (int k = 0; k < Ns; ++k) push(-2.0 * M_PI * double(int64_t(q) * k) / double(int64_t(Ns) * P));
         pl->wp[s] = int(tab.size() / 2);
         if (!fixed_radix(P))
             for (int m = 0; m < P; ++m) push(-2.0 * M_PI * double(m) / double(P));
@@ -1059,17 +1315,17 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     // plus two line buffers.  Lines that do not fit take the four-step path,
     // as complex transforms of the full length (all Fl bins kept).
     const size_t row_tw = size_t(ntw_of(L, mode == 0 ? Fl : 0));
-    const bool big = (row_tw + size_t(2) * L) * es > lds;
+    const bool big = (row_tw + size_t(2) * pp_len(L)) * es > lds;
     const int H = big ? Fl : (kComplex ? Fl : Fl / 2 + 1);
     int L1 = 0, L2 = 0, Ctb = 0;
     if (big) {
         for (int d2 = Fl; d2 >= 2; --d2) {
-            if (Fl % d2 || (size_t(ntw_of(d2, 0)) + size_t(3) * d2) * es > lds) continue;
+            if (Fl % d2 || (size_t(ntw_of(d2, 0)) + size_t(3) * pp_len(d2)) * es > lds) continue;
             const int d1 = Fl / d2;
             const size_t t1 = size_t(ntw_of(d1, 0));
             int ct = 16;
-            while (ct > 1 && (t1 + size_t(2) * d1 * ct) * es > lds) ct /= 2;
-            if ((t1 + size_t(2) * d1 * ct) * es > lds) continue;
+            while (ct > 1 && (t1 + size_t(2) * pp_len(d1) * ct) * es > lds) ct /= 2;
+            if ((t1 + size_t(2) * pp_len(d1) * ct) * es > lds) continue;
             L1 = d1;
             L2 = d2;
             Ctb = ct;
@@ -1085,9 +1341,6 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         ctx.err = "twiddle table allocation failed";
         return NDC_ERR_OUT_OF_MEMORY;
     }
-    int64_t offs[kMaxRank + 1];
-    const int32_t* maps = ctx.border_maps(p, offs);
-    if (!maps) return NDC_ERR_OUT_OF_MEMORY;
 
     PadSrc<E> pad;
     std::memset(&pad, 0, sizeof(pad));
@@ -1097,7 +1350,9 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     ker.k = static_cast<const E*>(kv);
     ker.rev = p.reverse ? 1 : 0;
     for (int i = 0; i < N; ++i) {
-        pad.map[i] = maps + offs[i];
+        pad.pds[i] = int32_t(p.pds[i]);
+        pad.side[i][0] = side1d(p.side[i][0].tag);
+        pad.side[i][1] = side1d(p.side[i][1].tag);
         pad.cst[i][0] = host_value<E>(p.side[i][0].value);
         pad.cst[i][1] = host_value<E>(p.side[i][1].value);
         pad.xstr[i] = p.x_str[i];
@@ -1118,7 +1373,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         return NDC_ERR_UNSUPPORTED;
     }
     // rank 1, line fits three buffers: one fused launch
-    if (!ks && N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * L) * es <= lds) {
+    if (!ks && N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * pp_len(L)) * es <= lds) {
         Fused1dArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
         a.pad = pad;
@@ -1130,7 +1385,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = (size_t(rowpl.ntw) + size_t(3) * L) * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(3) * pp_len(L)) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.k_numel + p.y_numel), 15.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
         hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
@@ -1153,9 +1408,11 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     }
     const AxisSel none{0, 1, 1};
 
+    // LDS of the row passes: the stage twiddles and two line buffers per line
+    const size_t row_tw_lds = big ? 0 : size_t(row_stage_tw(rowpl));
     auto rows_lpb = [&]() {
         int lpb = int(std::max<int64_t>(1, std::min<int64_t>(kMaxLpb, 512 / std::max(1, L))));
-        while (lpb > 1 && (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es > lds) lpb /= 2;
+        while (lpb > 1 && (row_tw_lds + size_t(2) * lpb * pp_len(L)) * es > lds) lpb /= 2;
         return lpb;
     };
 
@@ -1181,10 +1438,10 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.F = Fl;
         a.H = H;
         a.mode = mode;
-        a.ls = L;
+        a.ls = pp_len(L);
         const int lpb = rows_lpb();
         a.lpb = lpb;
-        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es;
+        const size_t smem = (row_tw_lds + size_t(2) * lpb * pp_len(L)) * es;
         a.pl = rowpl;
         const int64_t nb0 = (a.nlines[0] + lpb - 1) / lpb;
         const int64_t nb1 = (a.nlines[1] + lpb - 1) / lpb;
@@ -1217,7 +1474,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * L) * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * pp_len(L)) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.y_numel) + double(es) * H, 10.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
         hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
@@ -1273,7 +1530,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.pl = pl1;
         const int64_t nb0 = a.nlines[0] * a.ntiles, nb1 = a.nlines[1] * a.ntiles;
         a.nblocks0 = int32_t(nb0);
-        const size_t smem = (size_t(pl1.ntw) + size_t(2) * L1 * Ctb) * es;
+        const size_t smem = (size_t(pl1.ntw) + size_t(2) * pp_len(L1) * Ctb) * es;
         const double lines = double(a.nlines[0] + a.nlines[1]);
         const double bytes = inv ? double(es) * lines * Fl + esz * double(p.y_numel)
                                  : esz * double(p.x_numel + p.k_numel) + double(es) * lines * Fl;
@@ -1309,7 +1566,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
             if (rmode == 0) a.nrows[1] *= a.sel[1][e].count;
         }
         a.nblocks0 = int32_t(a.nrows[0]);
-        const size_t smem = (size_t(pl2.ntw) + size_t(rmode == 1 ? 3 : 2) * L2) * es;
+        const size_t smem = (size_t(pl2.ntw) + size_t(rmode == 1 ? 3 : 2) * pp_len(L2)) * es;
         const double rows = double(a.nrows[0] + a.nrows[1]);
         const double bytes = double(es) * L2 * (rmode == 1 ? 3.0 * a.nrows[0] : 2.0 * rows);
         static const char* names[3] = {"fft_bigrow_fwd", "fft_bigrow_fused", "fft_bigrow_inv"};
@@ -1380,7 +1637,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const int nbuf = (cmode == 1 || cmode == 3) ? 3 : 2;
         // tile width: enough lines for coalescing, enough blocks to fill the chip
         int Ct = 16;
-        auto smem_of = [&](int ct) { return (size_t(a.pl.ntw) + size_t(nbuf) * Ld * ct) * es; };
+        auto smem_of = [&](int ct) { return (size_t(a.pl.ntw) + size_t(nbuf) * pp_len(Ld) * ct) * es; };
         while (Ct > 1 && smem_of(Ct) > std::min<size_t>(lds, 96 * 1024)) Ct /= 2;
         // 8 columns = 64-byte segments per row (f32): below that, partial-line
         // reads and writes cost more than the extra blocks gain
@@ -1454,13 +1711,13 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.F = Fl;
         a.H = H;
         a.mode = mode;
-        a.ls = L;
+        a.ls = pp_len(L);
         const int lpb = rows_lpb();
         a.lpb = lpb;
         a.lastsel = AxisSel{p.kdil[last] - 1, p.stride[last], p.out[last]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = (size_t(rowpl.ntw) + size_t(2) * lpb * L) * es;
+        const size_t smem = (row_tw_lds + size_t(2) * lpb * pp_len(L)) * es;
         const int64_t nb = (a.nlines + lpb - 1) / lpb;
         const double bytes = double(es) * double(a.nlines) * H + esz * double(p.y_numel);
         ctx.kbegin("fft_rows_inv", bytes, 5.0 * double(a.nlines) * L * std::log2(double(L)));
@@ -1504,14 +1761,15 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
     extern __shared__ unsigned char smem_raw[];
     const int L = a.pl.L;
     const int Ct = a.Ct;
+    const int PLl = pp_len(L);   // LDS slots per line
     C* tw = reinterpret_cast<C*>(smem_raw);
     C* b0 = tw + a.pl.ntw;
-    C* b1 = b0 + L * Ct;
+    C* b1 = b0 + PLl * Ct;
     const int T = blockDim.x;
     const bool inv = a.inv != 0;
-    stage_table(tw, a.pl.tw, a.pl.ntw);
+    stage_table(tw, a.pl.tw, a.pl.ntw, T);
     if (a.inner > 1) {
-        // columns: Ct lines at consecutive inner indices, element (c, j) at j*Ct + c
+        // columns: Ct lines at consecutive inner indices, element (c, j) at pp(j)*Ct + c
         const int64_t o = blockIdx.x / a.ntiles;
         const int64_t i0 = int64_t(blockIdx.x % a.ntiles) * Ct;
         const int nc = int(imin64(Ct, a.inner - i0));
@@ -1531,18 +1789,21 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int it = b + u * T;
-                if (it < L * Ct) b0[it] = v[u];
+                if (it < L * Ct) {
+                    const int j = fdiv(it, Ct, rC), c = it - j * Ct;
+                    b0[pp(j) * Ct + c] = v[u];
+                }
             }
         }
         __syncthreads();
-        const C* res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, inv) ? b1 : b0;
+        const C* res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, inv, T) ? b1 : b0;
         for (int it = threadIdx.x; it < L * Ct; it += T) {
             const int j = fdiv(it, Ct, rC), c = it - j * Ct;
-            if (c < nc) a.out[base + int64_t(j) * a.inner + c] = cscale(res[it], a.scale);
+            if (c < nc) a.out[base + int64_t(j) * a.inner + c] = cscale(res[pp(j) * Ct + c], a.scale);
         }
         return;
     }
-    // rows: Ct consecutive rows, element (b, j) at b*L + j
+    // rows: Ct consecutive rows, element (b, j) at b*PLl + pp(j)
     const int64_t r0 = int64_t(blockIdx.x) * Ct;
     const int nr = int(imin64(Ct, a.outer - r0));
     const bool half = a.mode != 0 && L * 2 == a.n;   // even real line: half-length transform
@@ -1586,23 +1847,26 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int it = b + u * T;
-            if (it < tot) b0[it] = v[u];
+            if (it < tot) {
+                const int rb = fdiv(it, L, rL), j = it - rb * L;
+                b0[rb * PLl + pp(j)] = v[u];
+            }
         }
     }
     __syncthreads();
-    const C* res = fft_lines<R, false>(b0, b1, a.pl, tw, Ct, L, 1, inv) ? b1 : b0;
+    const C* res = fft_lines<R, false>(b0, b1, a.pl, tw, Ct, PLl, 1, inv, T) ? b1 : b0;
     if (a.mode == 0) {
         for (int it = threadIdx.x; it < tot; it += T) {
             const int rb = fdiv(it, L, rL), j = it - rb * L;
-            if (rb < nr) a.out[(r0 + rb) * a.n + j] = cscale(res[it], a.scale);
+            if (rb < nr) a.out[(r0 + rb) * a.n + j] = cscale(res[rb * PLl + pp(j)], a.scale);
         }
     } else if (a.mode == 1) {
         const int Hb = a.H;
         for (int it = threadIdx.x; it < Ct * Hb; it += T) {
             const int rb = it / Hb, k = it - rb * Hb;
             if (rb >= nr) continue;
-            const C* Z = res + rb * L;
-            const C X = half ? r2c_bin(Z, L, k, tw + a.pl.post) : Z[k];
+            const C* Z = res + rb * PLl;
+            const C X = half ? r2c_bin(Z, L, k, tw + a.pl.post) : Z[pp(k)];
             a.out[(r0 + rb) * Hb + k] = cscale(X, a.scale);
         }
     } else {
@@ -1610,11 +1874,12 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
             const int rb = fdiv(it, L, rL), j = it - rb * L;
             if (rb >= nr) continue;
             R* y = a.rout + (r0 + rb) * a.n;
+            const C zj = res[rb * PLl + pp(j)];
             if (half) {
-                y[2 * j] = res[it].x * a.scale;
-                y[2 * j + 1] = res[it].y * a.scale;
+                y[2 * j] = zj.x * a.scale;
+                y[2 * j + 1] = zj.y * a.scale;
             } else {
-                y[j] = res[it].x * a.scale;
+                y[j] = zj.x * a.scale;
             }
         }
     }
@@ -1666,16 +1931,16 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
     int64_t blocks = 0;
     if (inner > 1) {
         Ct = int(std::min<int64_t>(16, inner));
-        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es > lds) Ct /= 2;
+        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * pp_len(L) * Ct) * es > lds) Ct /= 2;
         a.ntiles = int32_t((inner + Ct - 1) / Ct);
         blocks = outer * a.ntiles;
     } else {
         Ct = int(std::max<int64_t>(1, std::min<int64_t>(kMaxLpb, 512 / std::max(1, L))));
-        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es > lds) Ct /= 2;
+        while (Ct > 1 && (size_t(a.pl.ntw) + size_t(2) * pp_len(L) * Ct) * es > lds) Ct /= 2;
         blocks = (outer + Ct - 1) / Ct;
     }
     a.Ct = Ct;
-    const size_t smem = (size_t(a.pl.ntw) + size_t(2) * L * Ct) * es;
+    const size_t smem = (size_t(a.pl.ntw) + size_t(2) * pp_len(L) * Ct) * es;
     if (smem > lds || !lds_fits(ctx, reinterpret_cast<const void*>(&line_fft_kernel<R>), smem)) {
         ctx.err = "line longer than the LDS transform supports";
         return NDC_ERR_UNSUPPORTED;
@@ -1791,7 +2056,7 @@ size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds) {
     const int64_t Fl = p.fft[last];
     const int mode = cplx ? 2 : (Fl % 2 == 0 ? 0 : 1);
     const int64_t L = mode == 0 ? Fl / 2 : Fl;
-    if ((size_t(ntw_of(int(L), mode == 0 ? int(Fl) : 0)) + size_t(2) * L) * es > lds) return 0;   // four-step rows
+    if ((size_t(ntw_of(int(L), mode == 0 ? int(Fl) : 0)) + size_t(2) * pp_len(int(L))) * es > lds) return 0;   // four-step rows
     const int64_t H = cplx ? Fl : Fl / 2 + 1;
     const int64_t Hp = (H + 7) / 8 * 8;
     int64_t n = Hp;
