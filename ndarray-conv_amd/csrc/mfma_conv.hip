@@ -62,18 +62,29 @@ struct MfmaArgs {
     int32_t npass_r;         // row groups of 32
 };
 
-// Value of the padded input from the border codes of its row and column
-// (codes resolved once per pass into LDS).
-__device__ __forceinline__ float pad_value(const MfmaArgs& a, int32_t cy, int32_t cx) {
+// Source of the padded input from the border codes of its row and column
+// (codes resolved once per pass into LDS): true with the offset into x, or
+// false with the cell's value in *fill.  No memory access, so that a batch's
+// loads can all be issued before any of them is used.
+__device__ __forceinline__ bool pad_source(const MfmaArgs& a, int32_t cy, int32_t cx, int64_t* off, float* fill) {
+    *off = 0;
+    *fill = 0.0f;
     // the higher axis' constant wins (later fills overwrite)
-    if (cx == kConstFront || cx == kConstBack) return a.cst[1][cx == kConstFront ? 0 : 1];
-    if (cy == kConstFront || cy == kConstBack) return a.cst[0][cy == kConstFront ? 0 : 1];
-    if (cx < 0 || cy < 0) return 0.0f;   // unwritten, or beyond the padded extent
-    return a.x[int64_t(cy) * a.xstr[0] + int64_t(cx) * a.xstr[1]];
+    if (cx == kConstFront || cx == kConstBack) {
+        *fill = a.cst[1][cx == kConstFront ? 0 : 1];
+        return false;
+    }
+    if (cy == kConstFront || cy == kConstBack) {
+        *fill = a.cst[0][cy == kConstFront ? 0 : 1];
+        return false;
+    }
+    if (cx < 0 || cy < 0) return false;   // unwritten, or beyond the padded extent
+    *off = int64_t(cy) * a.xstr[0] + int64_t(cx) * a.xstr[1];
+    return true;
 }
 
 template <int NKB, int Q, bool X3>
-__global__ void __launch_bounds__(kThreads) mfma_conv_kernel(MfmaArgs a) {
+__global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     constexpr int kSR = rows_per_batch<Q>();
     extern __shared__ uint32_t rows_lds[];   // [2][kSR][W] packed (hi, lo)
     __shared__ int32_t s_row[kMaxChain];     // border code of chain row t
@@ -150,32 +161,44 @@ __global__ void __launch_bounds__(kThreads) mfma_conv_kernel(MfmaArgs a) {
             }
             __syncthreads();
             const float rW = 1.0f / float(a.W);
-            auto fetch = [&](int b, uint32_t* regs) {
+            // Staging of a batch in two halves: fetch() computes every source
+            // and issues all loads without using them; the values are selected,
+            // split and written to LDS by store() after the next batch's
+            // compute, so the loads overlap it instead of waiting one by one.
+            float raw[kPer], fill[kPer];
+            uint32_t okm = 0;
+            auto fetch = [&](int b) {
+                int64_t off[kPer];
+                okm = 0;
 #pragma unroll
                 for (int i = 0; i < kPer; ++i) {
                     const int e = threadIdx.x + i * kThreads;
-                    uint32_t v = 0;
-                    if (e < kSR * a.W) {
-                        int rr = int(float(e) * rW);
-                        int cc = e - rr * a.W;
-                        if (cc < 0) { --rr; cc += a.W; } else if (cc >= a.W) { ++rr; cc -= a.W; }
-                        const int t = b * kSR + rr;
-                        if (t < nsteps) v = split_pack(pad_value(a, s_row[t], s_col[cc]));
-                    }
-                    regs[i] = v;
+                    off[i] = 0;
+                    fill[i] = 0.0f;
+                    int rr = int(float(e) * rW);
+                    int cc = e - rr * a.W;
+                    if (cc < 0) { --rr; cc += a.W; } else if (cc >= a.W) { ++rr; cc -= a.W; }
+                    const int t = b * kSR + rr;
+                    if (e < kSR * a.W && t < nsteps && pad_source(a, s_row[t], s_col[cc], &off[i], &fill[i]))
+                        okm |= 1u << i;
+                }
+#pragma unroll
+                for (int i = 0; i < kPer; ++i) raw[i] = a.x[off[i]];   // offset 0 (a valid element) where not used
+            };
+            auto store = [&](uint32_t* dst) {
+#pragma unroll
+                for (int i = 0; i < kPer; ++i) {
+                    const int e = threadIdx.x + i * kThreads;
+                    if (e < kSR * a.W) dst[e] = split_pack((okm >> i & 1u) ? raw[i] : fill[i]);
                 }
             };
-            uint32_t regs[kPer];   // host keeps W <= kMaxW, so kSR * W <= kPer * kThreads
-            fetch(0, regs);
-#pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const int e = threadIdx.x + i * kThreads;
-                if (e < kSR * a.W) rows_lds[e] = regs[i];
-            }
+            // host keeps W <= kMaxW, so kSR * W <= kPer * kThreads
+            fetch(0);
+            store(rows_lds);
             __syncthreads();
             int t = 0;
             for (int b = 0; b < nbatch; ++b) {
-                if (b + 1 < nbatch) fetch(b + 1, regs);
+                if (b + 1 < nbatch) fetch(b + 1);
                 const uint32_t* buf = rows_lds + (b & 1) * (kSR * a.W);
                 const int tend = min(nsteps, (b + 1) * kSR);
                 for (; t < tend;) {
@@ -232,14 +255,7 @@ __global__ void __launch_bounds__(kThreads) mfma_conv_kernel(MfmaArgs a) {
                         }
                     }
                 }
-                if (b + 1 < nbatch) {
-                    uint32_t* nbuf = rows_lds + ((b + 1) & 1) * (kSR * a.W);
-#pragma unroll
-                    for (int i = 0; i < kPer; ++i) {
-                        const int e = threadIdx.x + i * kThreads;
-                        if (e < kSR * a.W) nbuf[e] = regs[i];
-                    }
-                }
+                if (b + 1 < nbatch) store(rows_lds + ((b + 1) & 1) * (kSR * a.W));
                 __syncthreads();
             }
         }
