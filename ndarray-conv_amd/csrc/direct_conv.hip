@@ -2,7 +2,8 @@
 // non-zero taps of the (reversed) kernel (conv/mod.rs:177-197,
 // dilation/mod.rs:34-60), with the padded input never materialised: each
 // workgroup stages the input region its output tile needs into LDS, resolving
-// every padded coordinate through the per-axis border maps on the way in.
+// every padded coordinate through the per-axis border maps (in closed form,
+// border1d.hpp) on the way in.
 //
 // Layout: the last two axes form the tile plane (a rank-1 problem gets a
 // virtual unit axis in front); lower axes are looped over, one kernel tap at
@@ -12,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "border1d.hpp"
 #include "device_common.hpp"
 #include "launch.hpp"
 
@@ -23,12 +25,19 @@ constexpr int kTileX = 64;           // outputs per tile row: one wavefront
 constexpr int kTileY = 16;           // tile rows
 constexpr int kItems = kTileY * kTileX / kThreads;
 
+// Region cells staged per thread per round (fewer for 16-byte elements, which
+// would otherwise spill).
+template <typename T> constexpr int staged_per_round() { return sizeof(T) > 8 ? 2 : 8; }
+
 template <typename T>
 struct DirectArgs {
     const T* x;
     const T* k;
     T* y;
-    const int32_t* map[kMaxRank];
+    int32_t lo[kMaxRank];       // front pad: padded cell p of the interior is x index p - lo
+    int32_t n[kMaxRank];        // input extent
+    int32_t pds[kMaxRank];      // padded extent
+    int32_t side[kMaxRank][2];  // Side1D of the front and back fill
     T cst[kMaxRank][2];
     int64_t xstr[kMaxRank];
     int64_t kstr[kMaxRank];
@@ -46,16 +55,38 @@ struct DirectArgs {
     int32_t ntx, nty;   // tiles per plane
 };
 
+// Border-map code of padded cell q of axis e (plan.cpp axis_map, position by
+// position): an index into x, or kUnwritten / kConstFront / kConstBack.
 template <typename T>
-__device__ __forceinline__ T resolve_plane(const DirectArgs<T>& a, int32_t cy, int32_t cx, int oc_axis, T ocval,
-                                           bool ounw, int64_t xoff) {
+__device__ __forceinline__ int32_t axis_code(const DirectArgs<T>& a, int e, int q) {
+    return resolve1d<int>(q, a.n[e], a.lo[e], a.side[e][0], a.side[e][1]);
+}
+
+// Source of a plane cell from its row and column codes: true with the offset
+// into x, or false with the cell's value in *fill.  No memory access, so that
+// the staging loop can issue all of a round's loads before using any.
+template <typename T>
+__device__ __forceinline__ bool plane_source(const DirectArgs<T>& a, int32_t cy, int32_t cx, int oc_axis, T ocval,
+                                             bool ounw, int64_t xoff, int64_t* off, T* fill) {
     const int ay = a.No - 2, ax = a.No - 1;
+    *off = 0;
+    *fill = Arith<T>::zero();
     // the highest axis whose fill wrote a constant wins (later fills overwrite)
-    if (cx <= kConstFront) return a.cst[ax][cx == kConstFront ? 0 : 1];
-    if (cy <= kConstFront) return a.cst[ay][cy == kConstFront ? 0 : 1];
-    if (oc_axis >= 0) return ocval;
-    if (cx == kUnwritten || cy == kUnwritten || ounw) return Arith<T>::zero();
-    return a.x[xoff + int64_t(cy) * a.xstr[ay] + int64_t(cx) * a.xstr[ax]];
+    if (cx <= kConstFront) {
+        *fill = a.cst[ax][cx == kConstFront ? 0 : 1];
+        return false;
+    }
+    if (cy <= kConstFront) {
+        *fill = a.cst[ay][cy == kConstFront ? 0 : 1];
+        return false;
+    }
+    if (oc_axis >= 0) {
+        *fill = ocval;
+        return false;
+    }
+    if (cx == kUnwritten || cy == kUnwritten || ounw) return false;
+    *off = xoff + int64_t(cy) * a.xstr[ay] + int64_t(cx) * a.xstr[ax];
+    return true;
 }
 
 template <typename T>
@@ -112,7 +143,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                 const int64_t t = rem % a.K[e];
                 rem /= a.K[e];
                 const int64_t pe = oo[e] * a.s[e] + t * a.d[e];
-                const int32_t c = a.map[e][pe];
+                const int32_t c = axis_code(a, e, int(pe));
                 if (c >= 0) xoff += int64_t(c) * a.xstr[e];
                 else if (c == kUnwritten) ounw = true;
                 else if (oc_axis < e) { oc_axis = e; ocval = a.cst[e][c == kConstFront ? 0 : 1]; }
@@ -129,18 +160,40 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                 const int64_t px0 = ox0 * sx + int64_t(kx0) * dx;
                 const int64_t leny = (a.out[ay] - 1) * sy + (a.K[ay] - 1) * dy + 1;  // padded extent used
                 const int64_t lenx = (a.out[ax] - 1) * sx + (a.K[ax] - 1) * dx + 1;
+                // Rounds of kSU cells per thread: every cell's source is worked
+                // out first, then all of the round's loads are issued, then the
+                // values are selected and written, so the loads overlap.
+                constexpr int kSU = staged_per_round<T>();
                 const int nreg = a.RY * a.RX;
-                for (int i = threadIdx.x; i < nreg; i += kThreads) {
-                    const int r = i / a.RX, c = i - r * a.RX;
-                    const int64_t py = py0 + int64_t(r) * a.gy;
-                    const int64_t px = px0 + int64_t(c) * a.gx;
-                    T v = Arith<T>::zero();
-                    if (py < leny && px < lenx) {
-                        const int32_t cy = a.map[ay][py];
-                        const int32_t cx = a.map[ax][px];
-                        v = resolve_plane(a, cy, cx, oc_axis, ocval, ounw, xoff);
+                const float rRX = 1.0f / float(a.RX);
+                for (int i0 = threadIdx.x; i0 < nreg; i0 += kSU * kThreads) {
+                    int64_t off[kSU];
+                    T fill[kSU];
+                    bool ok[kSU];
+#pragma unroll
+                    for (int u = 0; u < kSU; ++u) {
+                        const int i = i0 + u * kThreads;
+                        const int ic = i < nreg ? i : 0;
+                        int r = int(float(ic) * rRX);
+                        int c = ic - r * a.RX;
+                        if (c < 0) { --r; c += a.RX; } else if (c >= a.RX) { ++r; c -= a.RX; }
+                        const int64_t py = py0 + int64_t(r) * a.gy;
+                        const int64_t px = px0 + int64_t(c) * a.gx;
+                        ok[u] = false;
+                        off[u] = 0;
+                        fill[u] = Arith<T>::zero();
+                        if (i < nreg && py < leny && px < lenx)
+                            ok[u] = plane_source(a, axis_code(a, ay, int(py)), axis_code(a, ax, int(px)), oc_axis,
+                                                 ocval, ounw, xoff, &off[u], &fill[u]);
                     }
-                    region[i] = v;
+                    T raw[kSU];
+#pragma unroll
+                    for (int u = 0; u < kSU; ++u) raw[u] = a.x[off[u]];   // offset 0 (a valid element) where !ok
+#pragma unroll
+                    for (int u = 0; u < kSU; ++u) {
+                        const int i = i0 + u * kThreads;
+                        if (i < nreg) region[i] = ok[u] ? raw[u] : fill[u];
+                    }
                 }
                 // stage the kernel taps of this chunk (value of reversed-view tap t)
                 for (int i = threadIdx.x; i < kyn * kxn; i += kThreads) {
@@ -188,13 +241,14 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
                         T (*conv_const)(const ndc_value&)) {
     DirectArgs<T> a;
     std::memset(&a, 0, sizeof(a));
-    int64_t offs[kMaxRank + 1];
-    const int32_t* maps = ctx.border_maps(p, offs);
-    if (!maps) return NDC_ERR_OUT_OF_MEMORY;
     const int shift = p.N == 1 ? 1 : 0;   // promote rank 1 to a unit-row plane
     a.No = p.N + shift;
     if (shift) {
-        a.map[0] = maps + offs[kMaxRank];
+        // a unit axis: the single cell is interior
+        a.lo[0] = 0;
+        a.n[0] = 1;
+        a.pds[0] = 1;
+        a.side[0][0] = a.side[0][1] = kS1Const;
         a.xstr[0] = 0;
         a.kstr[0] = 0;
         a.s[0] = 1;
@@ -205,7 +259,11 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     }
     for (int i = 0; i < p.N; ++i) {
         const int j = i + shift;
-        a.map[j] = maps + offs[i];
+        a.lo[j] = int32_t(p.pad[i][0]);
+        a.n[j] = int32_t(p.x_shape[i]);
+        a.pds[j] = int32_t(p.pds[i]);
+        a.side[j][0] = side1d(p.side[i][0].tag);
+        a.side[j][1] = side1d(p.side[i][1].tag);
         a.xstr[j] = p.x_str[i];
         a.kstr[j] = p.k_str[i];
         a.s[j] = p.stride[i];
