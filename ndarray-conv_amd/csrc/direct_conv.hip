@@ -29,6 +29,28 @@ constexpr int kItems = kTileY * kTileX / kThreads;
 // would otherwise spill).
 template <typename T> constexpr int staged_per_round() { return sizeof(T) > 8 ? 2 : 8; }
 
+// Kernel columns per register window of the sliding form.
+constexpr int kCX = 8;
+
+// A value every lane holds identically (a broadcast LDS read), moved to
+// scalar registers: the zero-tap test becomes a scalar branch and the
+// multiply takes a scalar operand.
+template <typename T>
+__device__ __forceinline__ T wave_uniform(T v) {
+    if constexpr (sizeof(T) < 4) {
+        return T(__builtin_amdgcn_readfirstlane(int(v)));
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "32-bit words");
+        constexpr int n = int(sizeof(T) / 4);
+        int w[n];
+        __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+        for (int i = 0; i < n; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+        __builtin_memcpy(&v, w, sizeof(T));
+        return v;
+    }
+}
+
 template <typename T>
 struct DirectArgs {
     const T* x;
@@ -89,7 +111,12 @@ __device__ __forceinline__ bool plane_source(const DirectArgs<T>& a, int32_t cy,
     return true;
 }
 
-template <typename T>
+// SLIDE (stride equal to dilation along the row, so consecutive outputs read
+// consecutive staged columns): each thread computes 4 adjacent outputs of
+// one tile row from a register window of kCX + 3 staged values per kCX
+// kernel columns, instead of one LDS read per product.  The taps of an
+// output are summed in the same order in both forms.
+template <typename T, bool SLIDE>
 __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) {
     extern __shared__ unsigned char smem_raw[];
     T* region = reinterpret_cast<T*>(smem_raw);
@@ -116,8 +143,9 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
         }
     }
     const int64_t oy0 = int64_t(ty) * kTileY, ox0 = int64_t(tx) * kTileX;
-    const int lx = threadIdx.x % kTileX;
-    const int ly0 = threadIdx.x / kTileX;
+    // plain form: column lx, rows ly0 + 4i; sliding form: row ly0, columns lx + i
+    const int lx = SLIDE ? (threadIdx.x % (kTileX / kItems)) * kItems : threadIdx.x % kTileX;
+    const int ly0 = SLIDE ? threadIdx.x / (kTileX / kItems) : threadIdx.x / kTileX;
 
     T acc[kItems];
 #pragma unroll
@@ -204,25 +232,60 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                     taps[i] = a.k[koff + kiy * a.kstr[ay] + kix * a.kstr[ax]];
                 }
                 __syncthreads();
-                for (int cy = 0; cy < kyn; ++cy) {
-                    const int roff = cy * dy_ * a.RX;
-                    for (int cx = 0; cx < kxn; ++cx) {
-                        const T v = taps[cy * kxn + cx];
-                        if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
-                        const int off = roff + cx * dx_;
+                if constexpr (SLIDE) {
+                    for (int cy = 0; cy < kyn; ++cy) {
+                        const T* rrow = region + (ly0 * sy_ + cy * dy_) * a.RX + lx;
+                        const T* trow = taps + cy * kxn;
+                        for (int cx0 = 0; cx0 < kxn; cx0 += kCX) {
+                            // window values this chunk uses: up to column lx + kxn + 2 < RX
+                            const int nw = min(kCX, kxn - cx0) + kItems - 1;
+                            T w[kCX + kItems - 1];
 #pragma unroll
-                        for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::mac(acc[i], region[base[i] + off], v);
+                            for (int i = 0; i < kCX + kItems - 1; ++i)
+                                w[i] = i < nw ? rrow[cx0 + i] : Arith<T>::zero();
+#pragma unroll
+                            for (int c = 0; c < kCX; ++c) {
+                                if (cx0 + c >= kxn) break;
+                                const T v = wave_uniform(trow[cx0 + c]);
+                                if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
+#pragma unroll
+                                for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
+                            }
+                        }
+                    }
+                } else {
+                    for (int cy = 0; cy < kyn; ++cy) {
+                        const int roff = cy * dy_ * a.RX;
+                        for (int cx = 0; cx < kxn; ++cx) {
+                            const T v = taps[cy * kxn + cx];
+                            if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
+                            const int off = roff + cx * dx_;
+#pragma unroll
+                            for (int i = 0; i < kItems; ++i)
+                                acc[i] = Arith<T>::mac(acc[i], region[base[i] + off], v);
+                        }
                     }
                 }
             }
         }
     }
-    const int64_t ox = ox0 + lx;
-    if (ox < a.out[ax]) {
+    if constexpr (SLIDE) {
+        const int64_t oy = oy0 + ly0;
+        if (oy < a.out[ay]) {
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
-            const int64_t oy = oy0 + ly0 + i * (kThreads / kTileX);
-            if (oy < a.out[ay]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
+            for (int i = 0; i < kItems; ++i) {
+                const int64_t ox = ox0 + lx + i;
+                if (ox < a.out[ax]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
+            }
+        }
+    } else {
+        const int64_t ox = ox0 + lx;
+        if (ox < a.out[ax]) {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) {
+                const int64_t oy = oy0 + ly0 + i * (kThreads / kTileX);
+                if (oy < a.out[ay]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
+            }
         }
     }
 }
@@ -324,8 +387,16 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     }
     ctx.kbegin("direct_conv", double(sizeof(T)) * double(p.x_numel + p.k_numel + p.y_numel),
                2.0 * double(p.y_numel) * double(p.k_numel));
-    if (!lds_fits(ctx, reinterpret_cast<const void*>(&direct_conv_kernel<T>), smem)) return NDC_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL(direct_conv_kernel<T>, dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(), a);
+    // the sliding form needs consecutive outputs on consecutive staged columns
+    const bool slide = a.s[ax] == a.d[ax];
+    const void* kp = slide ? reinterpret_cast<const void*>(&direct_conv_kernel<T, true>)
+                           : reinterpret_cast<const void*>(&direct_conv_kernel<T, false>);
+    if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
+    if (slide)
+        hipLaunchKernelGGL((direct_conv_kernel<T, true>), dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(), a);
+    else
+        hipLaunchKernelGGL((direct_conv_kernel<T, false>), dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(),
+                           a);
     ctx.kend();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
