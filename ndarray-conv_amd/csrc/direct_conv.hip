@@ -30,7 +30,7 @@ constexpr int kItems = kTileY * kTileX / kThreads;
 template <typename T> constexpr int staged_per_round() { return sizeof(T) > 8 ? 2 : 8; }
 
 // Kernel columns per register window of the sliding form.
-constexpr int kCX = 8;
+constexpr int kCX = 16;
 
 // A value every lane holds identically (a broadcast LDS read), moved to
 // scalar registers: the zero-tap test becomes a scalar branch and the
