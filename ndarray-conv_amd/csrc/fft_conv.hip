@@ -843,37 +843,53 @@ __global__ void __launch_bounds__(kMaxThreads) rows_inv_kernel(RowsInvArgs<R, E>
     const C* wpost = a.pl.tw + a.pl.post;
     const float rL = 1.0f / float(L);
     const int total = nl * L;
+    // Rounds of kU bins per thread: every source index first, then all of the
+    // round's loads, then the pre-processing and the LDS writes, so that no
+    // load waits behind another.
     for (int base = threadIdx.x; base < total; base += kU * T) {
-        C z[kU];
+        int at[kU], kk[kU], i1[kU], i2[kU];
+        const C* X[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int it = base + u * T;
-            z[u] = cmake<C>(R(0), R(0));
-            if (it < total) {
-                const int b = fdiv(it, L, rL), k = it - b * L;
-                const C* X = a.spec + s_off[b];
-                if (a.mode == 0) {
-                    // Im X[0] and Im X[M] are ignored as a c2r transform does
-                    C A = X[k], B = cconj(X[M - k]);
-                    if (k == 0) { A.y = 0; B.y = 0; }
-                    z[u] = c2r_pre(A, B, k, wpost);
-                } else if (a.mode == 1) {
-                    // odd F: Hermitian completion of the half spectrum
-                    if (k == 0) z[u] = cmake<C>(X[0].x, R(0));
-                    else if (k < a.H) z[u] = X[k];
-                    else z[u] = cconj(X[L - k]);
-                } else {
-                    z[u] = X[k];
-                }
+            const int itc = it < total ? it : 0;
+            const int b = fdiv(itc, L, rL), k = itc - b * L;
+            at[u] = b * a.ls + pp(k);
+            kk[u] = k;
+            X[u] = a.spec + s_off[b];
+            // mode 0: bins k and M - k; mode 1 (odd F): the Hermitian partner
+            // of a bin beyond the stored half; mode 2: bin k
+            i1[u] = (a.mode == 1 && k >= a.H) ? L - k : k;
+            i2[u] = M - k;
+        }
+        C A[kU], B[kU], W[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) A[u] = X[u][i1[u]];
+        if (a.mode == 0) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                B[u] = X[u][i2[u]];
+                W[u] = wpost[kk[u]];
             }
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int it = base + u * T;
-            if (it < total) {
-                const int b = fdiv(it, L, rL), k = it - b * L;
-                b0[b * a.ls + pp(k)] = z[u];
+            if (base + u * T >= total) continue;
+            const int k = kk[u];
+            C z;
+            if (a.mode == 0) {
+                // Im X[0] and Im X[M] are ignored as a c2r transform does
+                C Au = A[u], Bu = cconj(B[u]);
+                if (k == 0) { Au.y = 0; Bu.y = 0; }
+                const C t = cmul(csub(Au, Bu), cconj(W[u]));
+                z = cadd(cadd(Au, Bu), cmul_i(t));
+            } else if (a.mode == 1) {
+                // odd F: Hermitian completion of the half spectrum
+                z = k == 0 ? cmake<C>(A[u].x, R(0)) : (k < a.H ? A[u] : cconj(A[u]));
+            } else {
+                z = A[u];
             }
+            b0[at[u]] = z;
         }
     }
     tab.end(tw, a.pl.tw, ntw, T);
