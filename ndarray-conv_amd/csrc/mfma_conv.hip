@@ -211,9 +211,16 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
 #pragma unroll
                         for (int kb = 0; kb < NKB; ++kb) {
                             const int base = (32 * wave + m) * sxg + (16 * kb + 8 * half) * dxg;
+                            // every staged column of the K-block exists (W covers
+                            // 16 * NKB kernel columns): the reads are unconditional
+                            // and only a partial last block selects
                             uint32_t p[8];
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) p[j] = (amask[kb] >> j & 1u) ? row[base + j * dxg] : 0u;
+                            for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
+                            if (16 * kb + 16 > KCg) {
+#pragma unroll
+                                for (int j = 0; j < 8; ++j) p[j] = (amask[kb] >> j & 1u) ? p[j] : 0u;
+                            }
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
                                 ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
@@ -243,14 +250,24 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                             }
                         }
                         acc[s] = c;
-                        // the lane of the last kernel row holds output row y
+                        // the lane of the last kernel row holds output row y; the
+                        // common case (first pass, tile inside the row) is plain
+                        // stores at fixed offsets
                         const int yo = y0 + t - (KHg - 1) * Q;
-                        if (m == KHg - 1 && yo >= y0 && yo < y0 + BYe) {
-                            float* yr = a.y + int64_t(yo) * a.OX;
+                        if (yo >= y0 && yo < y0 + BYe && m == KHg - 1) {
+                            float* yr = a.y + int64_t(yo) * a.OX + xw + 4 * half;
+                            if (first && xw + 32 <= a.OX) {
 #pragma unroll
-                            for (int i = 0; i < 16; ++i) {
-                                const int xo = xw + 8 * (i / 4) + 4 * half + (i % 4);
-                                if (xo < a.OX) yr[xo] = first ? c[i] : yr[xo] + c[i];
+                                for (int i = 0; i < 16; ++i) yr[8 * (i / 4) + (i % 4)] = c[i];
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < 16; ++i) {
+                                    const int xo = xw + 8 * (i / 4) + 4 * half + (i % 4);
+                                    if (xo < a.OX) {
+                                        float* yp = yr + 8 * (i / 4) + (i % 4);
+                                        *yp = first ? c[i] : *yp + c[i];
+                                    }
+                                }
                             }
                         }
                     }
