@@ -83,7 +83,10 @@ __device__ __forceinline__ bool pad_source(const MfmaArgs& a, int32_t cy, int32_
     return true;
 }
 
-template <int NKB, int Q, bool X3>
+// D1: the subsampled column dilation is 1 (stride equal to dilation, or both
+// 1), so a K-block's 8 operand columns are consecutive staged entries and
+// their reads take fixed offsets from one address.
+template <int NKB, int Q, bool X3, bool D1>
 __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     constexpr int kSR = rows_per_batch<Q>();
     extern __shared__ uint32_t rows_lds[];   // [2][kSR][W] packed (hi, lo)
@@ -98,7 +101,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     const int x0 = tx * kXB;
     const int y0 = ty * a.BY;
     const int xw = x0 + 32 * wave;           // this wave's first output column
-    const int sxg = a.sx / a.gx, dxg = a.dx / a.gx;
+    const int sxg = a.sx / a.gx, dxg = D1 ? 1 : a.dx / a.gx;
     const int BYe = min(a.BY, a.OY - y0);
 
     for (int pr = 0; pr < a.npass_r; ++pr) {
@@ -131,15 +134,16 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                     bl[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
                 }
             }
-            // A-column validity: kernel columns beyond this pass contribute zero
-            uint32_t amask[NKB];
+            // A-column validity: kernel columns beyond this pass contribute
+            // zero; as AND masks on the packed BF16 pairs (columns 2i, 2i+1)
+            u32x4 amask[NKB];
 #pragma unroll
             for (int kb = 0; kb < NKB; ++kb) {
-                uint32_t mk = 0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (16 * kb + 8 * half + j < KCg) mk |= 1u << j;
-                amask[kb] = mk;
+                for (int i = 0; i < 4; ++i) {
+                    const int c = 16 * kb + 8 * half + 2 * i;
+                    amask[kb][i] = (c < KCg ? 0x0000ffffu : 0u) | (c + 1 < KCg ? 0xffff0000u : 0u);
+                }
             }
             floatx16 acc[Q];
 #pragma unroll
@@ -217,14 +221,18 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                             uint32_t p[8];
 #pragma unroll
                             for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
-                            if (16 * kb + 16 > KCg) {
-#pragma unroll
-                                for (int j = 0; j < 8; ++j) p[j] = (amask[kb] >> j & 1u) ? p[j] : 0u;
-                            }
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
                                 ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
                                 al[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                            }
+                            if (16 * kb + 16 > KCg) {
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    const uint32_t mk = amask[kb][i];
+                                    ah[kb][i] &= mk;
+                                    al[kb][i] &= mk;
+                                }
                             }
                         }
                         // shift the accumulator one kernel row up; row 0 starts fresh
@@ -288,27 +296,31 @@ int64_t gcd64(int64_t a, int64_t b) {
     return a;
 }
 
-template <int NKB, int Q, bool X3>
+template <int NKB, int Q, bool X3, bool D1>
 const void* kernel_ptr() {
-    return reinterpret_cast<const void*>(&mfma_conv_kernel<NKB, Q, X3>);
+    return reinterpret_cast<const void*>(&mfma_conv_kernel<NKB, Q, X3, D1>);
 }
 
-template <int NKB, int Q, bool X3>
+template <int NKB, int Q, bool X3, bool D1>
 void launch_one(const MfmaArgs& a, unsigned blocks, size_t smem, hipStream_t s) {
-    hipLaunchKernelGGL((mfma_conv_kernel<NKB, Q, X3>), dim3(blocks), dim3(kThreads), smem, s, a);
+    hipLaunchKernelGGL((mfma_conv_kernel<NKB, Q, X3, D1>), dim3(blocks), dim3(kThreads), smem, s, a);
 }
 
 template <bool X3>
-const void* pick(int nkb, int q, void (**fn)(const MfmaArgs&, unsigned, size_t, hipStream_t)) {
-#define NDC_MFMA_CASE(NK, QQ)                          \
-    if (nkb == NK && q == QQ) {                        \
-        *fn = &launch_one<NK, QQ, X3>;                 \
-        return kernel_ptr<NK, QQ, X3>();               \
+const void* pick(int nkb, int q, bool d1, void (**fn)(const MfmaArgs&, unsigned, size_t, hipStream_t)) {
+#define NDC_MFMA_CASE(NK, QQ, DD)                        \
+    if (nkb == NK && q == QQ && d1 == DD) {              \
+        *fn = &launch_one<NK, QQ, X3, DD>;               \
+        return kernel_ptr<NK, QQ, X3, DD>();             \
     }
-    NDC_MFMA_CASE(1, 1) NDC_MFMA_CASE(2, 1) NDC_MFMA_CASE(3, 1) NDC_MFMA_CASE(4, 1)
-    NDC_MFMA_CASE(1, 2) NDC_MFMA_CASE(2, 2) NDC_MFMA_CASE(3, 2) NDC_MFMA_CASE(4, 2)
-    NDC_MFMA_CASE(1, 3) NDC_MFMA_CASE(2, 3) NDC_MFMA_CASE(3, 3) NDC_MFMA_CASE(4, 3)
-    NDC_MFMA_CASE(1, 4) NDC_MFMA_CASE(2, 4) NDC_MFMA_CASE(3, 4) NDC_MFMA_CASE(4, 4)
+    // consecutive operand columns only for the single-set chain (row stride
+    // equal to row dilation), which covers the common shapes
+    NDC_MFMA_CASE(1, 1, true) NDC_MFMA_CASE(2, 1, true) NDC_MFMA_CASE(3, 1, true) NDC_MFMA_CASE(4, 1, true)
+    if (d1) d1 = false;
+    NDC_MFMA_CASE(1, 1, false) NDC_MFMA_CASE(2, 1, false) NDC_MFMA_CASE(3, 1, false) NDC_MFMA_CASE(4, 1, false)
+    NDC_MFMA_CASE(1, 2, false) NDC_MFMA_CASE(2, 2, false) NDC_MFMA_CASE(3, 2, false) NDC_MFMA_CASE(4, 2, false)
+    NDC_MFMA_CASE(1, 3, false) NDC_MFMA_CASE(2, 3, false) NDC_MFMA_CASE(3, 3, false) NDC_MFMA_CASE(4, 3, false)
+    NDC_MFMA_CASE(1, 4, false) NDC_MFMA_CASE(2, 4, false) NDC_MFMA_CASE(3, 4, false) NDC_MFMA_CASE(4, 4, false)
 #undef NDC_MFMA_CASE
     return nullptr;
 }
@@ -391,7 +403,8 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     const size_t smem = size_t(2) * SR * a.W * sizeof(uint32_t);
     void (*fn)(const MfmaArgs&, unsigned, size_t, hipStream_t) = nullptr;
     const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
-    const void* kp = x3 ? pick<true>(nkb, Q, &fn) : pick<false>(nkb, Q, &fn);
+    const bool d1 = dx / a.gx == 1;
+    const void* kp = x3 ? pick<true>(nkb, Q, d1, &fn) : pick<false>(nkb, Q, d1, &fn);
     if (!kp || !fn) return NDC_ERR_UNSUPPORTED;
     if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
     // algorithmic work: 2 flops per output and tap (the FP32-equivalent count)
