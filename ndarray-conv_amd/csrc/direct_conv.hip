@@ -304,6 +304,12 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
                         T (*conv_const)(const ndc_value&)) {
     DirectArgs<T> a;
     std::memset(&a, 0, sizeof(a));
+    // the closed-form border codes are evaluated in 32-bit positions
+    for (int i = 0; i < p.N; ++i)
+        if (p.pds[i] > INT32_MAX) {
+            ctx.err = "direct convolution: a padded axis longer than 2^31 - 1 cells";
+            return NDC_ERR_UNSUPPORTED;
+        }
     const int shift = p.N == 1 ? 1 : 0;   // promote rank 1 to a unit-row plane
     a.No = p.N + shift;
     if (shift) {
