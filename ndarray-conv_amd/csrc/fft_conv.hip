@@ -66,13 +66,15 @@ struct LinePlan {
     int wp[kMaxStages];     // radix-P roots W_P^m at wp + m (stages without a fixed butterfly)
 };
 
-// LDS layout of a line: element t sits at pp(t), one spare slot after every
-// 16 elements.  The early Stockham stages write with stride P and column tiles
-// keep rows 16*Ct apart; without the padding those accesses fall into a few
-// banks (up to 16-way conflicts on the first radix-8 stage of a row).
-__host__ __device__ __forceinline__ int pp(int t) { return t + (t >> 4); }
+// LDS layout of a line: element t sits at slot pp(t).  Every line access goes
+// through this pair, so the layout is decided here.  It is the plain one: a
+// padded layout (one spare slot after every 16 elements) removes the bank
+// conflicts of the early Stockham stages' stride-P writes, but on gfx950 the
+// extra address arithmetic cost more than the conflicts (measured on the 1-D,
+// 2-D and 3-D workloads: 6% slower end to end).
+__host__ __device__ __forceinline__ int pp(int t) { return t; }
 // LDS slots a line of L elements spans in that layout.
-__host__ __device__ __forceinline__ int pp_len(int L) { return L + (L >> 4) + 1; }
+__host__ __device__ __forceinline__ int pp_len(int L) { return L; }
 
 // ---------------------------------------------------------------------------
 // Stockham stages over lines in LDS: element (line b, index t) at b*ls + pp(t)*es.
