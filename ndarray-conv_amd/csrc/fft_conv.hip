@@ -83,7 +83,11 @@ __host__ __device__ __forceinline__ int pp_len(int L) { return L; }
 // fastest for rows (es = 1).  `tw` points at the plan's table in LDS.
 // ---------------------------------------------------------------------------
 
-template <typename R, int P, bool LINE_FAST, bool INV>
+// SHIFT: the item splits are powers of two (Ns, and the line count or the
+// butterflies per line) or the block holds a single line, so they take shifts
+// and masks instead of the reciprocal divisions; the stages are bound by the
+// vector instructions of the busiest SIMD, so the index arithmetic counts.
+template <typename R, int P, bool LINE_FAST, bool INV, bool SHIFT>
 __device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst, int L, int Ns,
                                             int nlines, int ls, int es, const Cx<R>* __restrict__ twb, int T) {
     using C = Cx<R>;
@@ -92,12 +96,23 @@ __device__ __forceinline__ void stage_fixed(const Cx<R>* __restrict__ src, Cx<R>
     const int dv = LINE_FAST ? nlines : nbf;
     const float rdv = 1.0f / float(dv);
     const float rns = 1.0f / float(Ns);
+    const int ldv = SHIFT ? __builtin_ctz(unsigned(dv)) : 0;
+    const int lns = SHIFT ? __builtin_ctz(unsigned(Ns)) : 0;
+    const bool one = !LINE_FAST && nlines == 1;
     for (int it = threadIdx.x; it < total; it += T) {
-        int b, j;
-        if (LINE_FAST) { j = fdiv(it, dv, rdv); b = it - j * dv; }
-        else { b = fdiv(it, dv, rdv); j = it - b * dv; }
-        const int q = fdiv(j, Ns, rns);
-        const int k = j - q * Ns;
+        int b, j, q, k;
+        if (SHIFT) {
+            if (LINE_FAST) { j = it >> ldv; b = it & (dv - 1); }
+            else if (one) { b = 0; j = it; }
+            else { b = it >> ldv; j = it & (dv - 1); }
+            q = j >> lns;
+            k = j & (Ns - 1);
+        } else {
+            if (LINE_FAST) { j = fdiv(it, dv, rdv); b = it - j * dv; }
+            else { b = fdiv(it, dv, rdv); j = it - b * dv; }
+            q = fdiv(j, Ns, rns);
+            k = j - q * Ns;
+        }
         const C* s = src + b * ls;
         // All LDS reads of the butterfly (data and twiddles) are issued before
         // any arithmetic: left to itself the scheduler interleaves the twiddle
@@ -173,14 +188,23 @@ __device__ __forceinline__ int fft_lines_dir(Cx<R>* b0, Cx<R>* b1, const LinePla
         const Cx<R>* src = cur ? b1 : b0;
         Cx<R>* dst = cur ? b0 : b1;
         const Cx<R>* twb = tw + pl.base[s];
+        const int dvs = LINE_FAST ? nlines : pl.L / P;
+        const bool shift = (Ns & (Ns - 1)) == 0 &&
+                           (((dvs & (dvs - 1)) == 0) || (!LINE_FAST && nlines == 1));
         switch (P) {
-            case 2: stage_fixed<R, 2, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 3: stage_fixed<R, 3, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 4: stage_fixed<R, 4, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 5: stage_fixed<R, 5, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 7: stage_fixed<R, 7, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 8: stage_fixed<R, 8, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
-            case 16: stage_fixed<R, 16, LINE_FAST, INV>(src, dst, pl.L, Ns, nlines, ls, es, twb, T); break;
+#define NDC_STAGE(PP)                                                                                     \
+    case PP:                                                                                              \
+        if (shift) stage_fixed<R, PP, LINE_FAST, INV, true>(src, dst, pl.L, Ns, nlines, ls, es, twb, T);  \
+        else stage_fixed<R, PP, LINE_FAST, INV, false>(src, dst, pl.L, Ns, nlines, ls, es, twb, T);       \
+        break;
+            NDC_STAGE(2)
+            NDC_STAGE(3)
+            NDC_STAGE(4)
+            NDC_STAGE(5)
+            NDC_STAGE(7)
+            NDC_STAGE(8)
+            NDC_STAGE(16)
+#undef NDC_STAGE
             default:
                 stage_generic<R, LINE_FAST, INV>(src, dst, pl.L, Ns, P, nlines, ls, es, twb, tw + pl.wp[s], T);
                 break;
