@@ -203,8 +203,13 @@ __device__ __forceinline__ int fft_lines_dir(Cx<R>* b0, Cx<R>* b1, const LinePla
             NDC_STAGE(5)
             NDC_STAGE(7)
             NDC_STAGE(8)
-            NDC_STAGE(16)
 #undef NDC_STAGE
+            case 16:
+                if constexpr (sizeof(R) == 4) {
+                    if (shift) stage_fixed<R, 16, LINE_FAST, INV, true>(src, dst, pl.L, Ns, nlines, ls, es, twb, T);
+                    else stage_fixed<R, 16, LINE_FAST, INV, false>(src, dst, pl.L, Ns, nlines, ls, es, twb, T);
+                }
+                break;
             default:
                 stage_generic<R, LINE_FAST, INV>(src, dst, pl.L, Ns, P, nlines, ls, es, twb, tw + pl.wp[s], T);
                 break;
@@ -1238,12 +1243,15 @@ __global__ void __launch_bounds__(kMaxThreads) bigrow_kernel(BigRowArgs<R> a) {
 // radix at most 16, with the bits spread evenly over them (2^8 = 16*16,
 // 2^9 = 8*8*8, 2^10 = 16*8*8): fewer stages mean fewer LDS round trips and
 // barriers, and radix 8-16 butterflies cost the fewest instructions per point.
-std::vector<int> factor_radices(int L) {
+// Double-precision plans stop at radix 8: a radix-16 butterfly of complex
+// doubles does not fit the registers of a 1024-thread block.
+std::vector<int> factor_radices(int L, bool dbl) {
     std::vector<int> r;
     int n = L, a = 0;
     while (n % 2 == 0) { n /= 2; ++a; }
     if (a > 0) {
-        const int st = (a + 3) / 4;
+        const int maxb = dbl ? 3 : 4;
+        const int st = (a + maxb - 1) / maxb;
         for (int i = 0; i < st; ++i) {
             const int bits = a / st + (i < a % st ? 1 : 0);
             r.push_back(1 << bits);
@@ -1260,9 +1268,9 @@ std::vector<int> factor_radices(int L) {
 bool fixed_radix(int p) { return p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16; }
 
 // Entries of the compact twiddle table make_lineplan builds for (L, post_F).
-int64_t ntw_of(int L, int post_F) {
+int64_t ntw_of(int L, int post_F, bool dbl) {
     int64_t n = 0, Ns = 1;
-    for (int P : factor_radices(L)) {
+    for (int P : factor_radices(L, dbl)) {
         n += Ns * (P - 1);
         if (!fixed_radix(P)) n += P;
         Ns *= P;
@@ -1284,7 +1292,7 @@ bool make_lineplan(LinePlan<R>* pl, int L, int post_F, LaunchContext& ctx) {
     }
     std::memset(pl, 0, sizeof(*pl));
     pl->L = L;
-    auto r = factor_radices(L);
+    auto r = factor_radices(L, sizeof(R) == 8);
     if (int(r.size()) > kMaxStages) return false;
     pl->nst = int(r.size());
     std::vector<double> tab;   // interleaved re, im
@@ -1356,15 +1364,16 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     // LDS of a row plan: the compact table (about L entries, 2L for real-even)
     // plus two line buffers.  Lines that do not fit take the four-step path,
     // as complex transforms of the full length (all Fl bins kept).
-    const size_t row_tw = size_t(ntw_of(L, mode == 0 ? Fl : 0));
+    constexpr bool kDbl = sizeof(R) == 8;
+    const size_t row_tw = size_t(ntw_of(L, mode == 0 ? Fl : 0, kDbl));
     const bool big = (row_tw + size_t(2) * pp_len(L)) * es > lds;
     const int H = big ? Fl : (kComplex ? Fl : Fl / 2 + 1);
     int L1 = 0, L2 = 0, Ctb = 0;
     if (big) {
         for (int d2 = Fl; d2 >= 2; --d2) {
-            if (Fl % d2 || (size_t(ntw_of(d2, 0)) + size_t(3) * pp_len(d2)) * es > lds) continue;
+            if (Fl % d2 || (size_t(ntw_of(d2, 0, kDbl)) + size_t(3) * pp_len(d2)) * es > lds) continue;
             const int d1 = Fl / d2;
-            const size_t t1 = size_t(ntw_of(d1, 0));
+            const size_t t1 = size_t(ntw_of(d1, 0, kDbl));
             int ct = 16;
             while (ct > 1 && (t1 + size_t(2) * pp_len(d1) * ct) * es > lds) ct /= 2;
             if ((t1 + size_t(2) * pp_len(d1) * ct) * es > lds) continue;
@@ -2098,7 +2107,8 @@ size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds) {
     const int64_t Fl = p.fft[last];
     const int mode = cplx ? 2 : (Fl % 2 == 0 ? 0 : 1);
     const int64_t L = mode == 0 ? Fl / 2 : Fl;
-    if ((size_t(ntw_of(int(L), mode == 0 ? int(Fl) : 0)) + size_t(2) * pp_len(int(L))) * es > lds) return 0;   // four-step rows
+    if ((size_t(ntw_of(int(L), mode == 0 ? int(Fl) : 0, es == 16)) + size_t(2) * pp_len(int(L))) * es > lds)
+        return 0;   // four-step rows
     const int64_t H = cplx ? Fl : Fl / 2 + 1;
     const int64_t Hp = (H + 7) / 8 * 8;
     int64_t n = Hp;
