@@ -226,7 +226,7 @@ def run_batch(args):
                          "circular")
         worst = max(worst, O.relerr(y[yo[i]:yo[i + 1]], ref)[0])
 
-    # dominant kernel: HIP events around each launch on the library's stream
+    # dominant kernel: HIP events carried by each launch (its dispatch's start/end)
     nc._check(lib.ndc_profile_enable(h, 1), h)
     for _ in range(args.steps):
         bp.run(xt, kt, yt)
@@ -377,7 +377,7 @@ def main():
     outputs = int(np.prod(out_shape))
     value = outputs / (ms * 1e-3)
 
-    # per-kernel durations: HIP events around every launch on the same stream
+    # per-kernel durations: HIP events carried by every launch (dispatch start/end)
     nc._check(lib.ndc_profile_enable(h, 1), h)
     for _ in range(args.steps):
         step()

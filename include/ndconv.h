@@ -230,8 +230,10 @@ ndc_status ndc_fft_backward(ndc_handle h, int32_t dtype, int32_t rank, const int
 ndc_status ndc_fft_origin_len(ndc_handle h, int64_t* len);
 
 /* Per-kernel profile (no reference counterpart: the reference has no
- * instrumentation).  While enabled, every kernel launch on the handle is
- * bracketed by HIP events on the handle's stream.  The report synchronizes
+ * instrumentation).  While enabled, every kernel launch on the handle carries
+ * a pair of HIP events that take its dispatch's start and end times
+ * (hipExtLaunchKernel; launches into a stream being captured are not
+ * recorded).  The report synchronizes
  * the stream and writes a JSON object
  *   {"<kernel>": {"launches": n, "total_ms": t, "bytes": b, "flops": f}, ...}
  * where bytes/flops are the summed algorithmic HBM traffic and arithmetic of

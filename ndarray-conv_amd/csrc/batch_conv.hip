@@ -631,7 +631,7 @@ ndc_status launch_batch(const BatchArgs& a, int grid, LaunchContext& ctx, hipStr
         return NDC_ERR_UNSUPPORTED;
     const int kgrid = a.nkitems > 0 ? a.nkitems : 1;
     if (profile) ctx.kbegin("batch_kspec", kspec_bytes, 0);
-    hipLaunchKernelGGL(batch_kspec_kernel, dim3(kgrid), dim3(kBatchThreads), kLdsBytes, stream, a);
+    NDC_LAUNCH(ctx, stream, batch_kspec_kernel, dim3(kgrid), dim3(kBatchThreads), kLdsBytes, a);
     if (profile) ctx.kend();
     if (hipGetLastError() != hipSuccess) {
         ctx.err = "batch_kspec launch failed";
@@ -639,7 +639,7 @@ ndc_status launch_batch(const BatchArgs& a, int grid, LaunchContext& ctx, hipStr
     }
     if (a.nitems > 0 && grid > 0) {
         if (profile) ctx.kbegin("batch_main", main_bytes, main_flops);
-        hipLaunchKernelGGL(batch_main_kernel, dim3(grid), dim3(kBatchThreads), kLdsBytes, stream, a);
+        NDC_LAUNCH(ctx, stream, batch_main_kernel, dim3(grid), dim3(kBatchThreads), kLdsBytes, a);
         if (profile) ctx.kend();
         if (hipGetLastError() != hipSuccess) {
             ctx.err = "batch_main launch failed";

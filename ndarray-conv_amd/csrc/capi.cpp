@@ -71,17 +71,40 @@ public:
         return e;
     }
 
+    // A bracket's events are handed to its launch (kevents), which records
+    // them at the dispatch's start and end; a bracket whose launch did not
+    // take them is dropped at kend.
+    bool armed = false;
+
     void kbegin(const char* name, double bytes, double flops) override {
+        armed = false;
         if (!prof) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream(), &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
         Rec r{name, bytes, flops, event(), event()};
-        if (!r.a || !r.b) return;
-        (void)hipEventRecord(r.a, stream());
+        if (!r.a || !r.b) {
+            if (r.a) pool.push_back(r.a);
+            if (r.b) pool.push_back(r.b);
+            return;
+        }
         recs.push_back(r);
+        armed = true;
+    }
+
+    bool kevents(hipEvent_t* a, hipEvent_t* b) override {
+        if (!armed) return false;
+        armed = false;
+        *a = recs.back().a;
+        *b = recs.back().b;
+        return true;
     }
 
     void kend() override {
-        if (!prof || recs.empty()) return;
-        (void)hipEventRecord(recs.back().b, stream());
+        if (!armed) return;
+        armed = false;
+        pool.push_back(recs.back().a);
+        pool.push_back(recs.back().b);
+        recs.pop_back();
     }
 
     hipStream_t stream() override { return use_ext ? ext : own; }

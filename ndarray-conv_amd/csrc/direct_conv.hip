@@ -399,10 +399,11 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
                            : reinterpret_cast<const void*>(&direct_conv_kernel<T, false>);
     if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
     if (slide)
-        hipLaunchKernelGGL((direct_conv_kernel<T, true>), dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, true>),
+            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
     else
-        hipLaunchKernelGGL((direct_conv_kernel<T, false>), dim3(unsigned(nblocks)), dim3(kThreads), smem, ctx.stream(),
-                           a);
+        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, false>),
+            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
     ctx.kend();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -1439,7 +1439,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const size_t smem = (size_t(rowpl.ntw) + size_t(3) * pp_len(L)) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.k_numel + p.y_numel), 15.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, a);
         ctx.kend();
         return check_launch(ctx, "fused rank-1 FFT convolution");
     }
@@ -1501,8 +1501,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
                              double(es) * double(a.nlines[0] + a.nlines[1]) * H;
         ctx.kbegin("fft_rows_fwd", bytes, 5.0 * double(a.nlines[0] + a.nlines[1]) * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_fwd_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((rows_fwd_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(lpb) * L)),
-                           smem, ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (rows_fwd_kernel<R, E>),
+            dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(lpb) * L)), smem, a);
         ctx.kend();
         return check_launch(ctx, "rows forward");
     };
@@ -1528,7 +1528,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const size_t smem = (size_t(rowpl.ntw) + size_t(2) * pp_len(L)) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.y_numel) + double(es) * H, 10.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, a);
         ctx.kend();
         return check_launch(ctx, "fused rank-1 FFT convolution");
     }
@@ -1587,8 +1587,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
                                  : esz * double(p.x_numel + p.k_numel) + double(es) * lines * Fl;
         ctx.kbegin(inv ? "fft_bigcol_inv" : "fft_bigcol_fwd", bytes, 5.0 * lines * Fl * std::log2(double(L1)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigcol_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((bigcol_kernel<R, E>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(L1) * Ctb)),
-                           smem, ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (bigcol_kernel<R, E>),
+            dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(L1) * Ctb)), smem, a);
         ctx.kend();
         return check_launch(ctx, "long-axis column pass");
     };
@@ -1623,8 +1623,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         static const char* names[3] = {"fft_bigrow_fwd", "fft_bigrow_fused", "fft_bigrow_inv"};
         ctx.kbegin(names[rmode], bytes, 5.0 * rows * (rmode == 1 ? 3 : 1) * L2 * std::log2(double(L2)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigrow_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((bigrow_kernel<R>), dim3(unsigned(a.nrows[0] + a.nrows[1])), dim3(threads_for(L2)), smem,
-                           ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (bigrow_kernel<R>),
+            dim3(unsigned(a.nrows[0] + a.nrows[1])), dim3(threads_for(L2)), smem, a);
         ctx.kend();
         return check_launch(ctx, "long-axis row pass");
     };
@@ -1714,8 +1714,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         static const char* names[4] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv", "fft_cols_fused_kept"};
         ctx.kbegin(names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&cols_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((cols_kernel<R>), dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(Ld) * Ct / 2)), smem,
-                           ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (cols_kernel<R>),
+            dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(Ld) * Ct / 2)), smem, a);
         ctx.kend();
         return check_launch(ctx, "column pass");
     };
@@ -1773,8 +1773,8 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const double bytes = double(es) * double(a.nlines) * H + esz * double(p.y_numel);
         ctx.kbegin("fft_rows_inv", bytes, 5.0 * double(a.nlines) * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&rows_inv_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
-        hipLaunchKernelGGL((rows_inv_kernel<R, E>), dim3(unsigned(nb)), dim3(threads_for(int64_t(lpb) * L)), smem,
-                           ctx.stream(), a);
+        NDC_LAUNCH(ctx, ctx.stream(), (rows_inv_kernel<R, E>),
+            dim3(unsigned(nb)), dim3(threads_for(int64_t(lpb) * L)), smem, a);
         ctx.kend();
         ndc_status s = check_launch(ctx, "rows inverse");
         if (s != NDC_OK) return s;
@@ -1999,8 +1999,8 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
     if (blocks <= 0) return NDC_OK;
     const double bytes = double(outer) * double(inner) * double(n) * double(es) * 2.0;
     ctx.kbegin("fft_processor_lines", bytes, 5.0 * double(outer) * double(inner) * L * std::log2(double(std::max(2, L))));
-    hipLaunchKernelGGL((line_fft_kernel<R>), dim3(unsigned(blocks)), dim3(threads_for(int64_t(L) * Ct)), smem,
-                       ctx.stream(), a);
+    NDC_LAUNCH(ctx, ctx.stream(), (line_fft_kernel<R>),
+        dim3(unsigned(blocks)), dim3(threads_for(int64_t(L) * Ct)), smem, a);
     ctx.kend();
     return check_launch(ctx, "processor line transform");
 }
@@ -2010,8 +2010,8 @@ ndc_status transpose(LaunchContext& ctx, const void* in, void* out, int64_t A, i
     if (A <= 0 || B <= 0) return NDC_OK;
     const dim3 grid(unsigned((B + 31) / 32), unsigned((A + 31) / 32));
     ctx.kbegin("fft_processor_transpose", 2.0 * double(A) * double(B) * double(sizeof(Cx<R>)), 0);
-    hipLaunchKernelGGL((transpose_kernel<R>), grid, dim3(32, 8), 0, ctx.stream(), static_cast<const Cx<R>*>(in),
-                       static_cast<Cx<R>*>(out), A, B);
+    NDC_LAUNCH(ctx, ctx.stream(), (transpose_kernel<R>),
+        grid, dim3(32, 8), 0, static_cast<const Cx<R>*>(in), static_cast<Cx<R>*>(out), A, B);
     ctx.kend();
     return check_launch(ctx, "transpose");
 }

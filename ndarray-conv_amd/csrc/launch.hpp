@@ -2,6 +2,7 @@
 // the .hip translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -42,10 +43,28 @@ public:
     // Bracket one kernel launch for the optional per-kernel profile: `name`
     // identifies the kernel, `bytes` is its algorithmic HBM traffic (compulsory
     // reads + writes of that pass) and `flops` its algorithmic arithmetic.
+    // The launch between them is made with NDC_LAUNCH, which asks kevents for
+    // the bracket's start/stop events (false: profile off, already taken, or
+    // the stream is being captured).
     virtual void kbegin(const char* name, double bytes, double flops) { (void)name; (void)bytes; (void)flops; }
+    virtual bool kevents(hipEvent_t* start, hipEvent_t* stop) { (void)start; (void)stop; return false; }
     virtual void kend() {}
     std::string err;
 };
+
+// Launches `kernel` (parenthesised when it names template arguments) on
+// `stream`.  Inside a profiled kbegin/kend bracket the launch carries the
+// bracket's events (hipExtLaunchKernelGGL), which then hold the dispatch's own
+// start and end times rather than those of marker packets queued around it.
+#define NDC_LAUNCH(ctx, stream, kernel, grid, block, smem, ...)                                           \
+    do {                                                                                                  \
+        hipEvent_t ndc_e0_ = nullptr, ndc_e1_ = nullptr;                                                  \
+        if ((ctx).kevents(&ndc_e0_, &ndc_e1_))                                                            \
+            hipExtLaunchKernelGGL(kernel, grid, block, uint32_t(smem), stream, ndc_e0_, ndc_e1_, 0,       \
+                                  __VA_ARGS__);                                                           \
+        else                                                                                              \
+            hipLaunchKernelGGL(kernel, grid, block, smem, stream, __VA_ARGS__);                           \
+    } while (0)
 
 // True when `kernel`'s static LDS plus `dynamic_bytes` fits one workgroup;
 // a launch that would not fit is refused on the host instead of reaching the

@@ -302,12 +302,12 @@ const void* kernel_ptr() {
 }
 
 template <int NKB, int Q, bool X3, bool D1>
-void launch_one(const MfmaArgs& a, unsigned blocks, size_t smem, hipStream_t s) {
-    hipLaunchKernelGGL((mfma_conv_kernel<NKB, Q, X3, D1>), dim3(blocks), dim3(kThreads), smem, s, a);
+void launch_one(const MfmaArgs& a, unsigned blocks, size_t smem, LaunchContext& ctx) {
+    NDC_LAUNCH(ctx, ctx.stream(), (mfma_conv_kernel<NKB, Q, X3, D1>), dim3(blocks), dim3(kThreads), smem, a);
 }
 
 template <bool X3>
-const void* pick(int nkb, int q, bool d1, void (**fn)(const MfmaArgs&, unsigned, size_t, hipStream_t)) {
+const void* pick(int nkb, int q, bool d1, void (**fn)(const MfmaArgs&, unsigned, size_t, LaunchContext&)) {
 #define NDC_MFMA_CASE(NK, QQ, DD)                        \
     if (nkb == NK && q == QQ && d1 == DD) {              \
         *fn = &launch_one<NK, QQ, X3, DD>;               \
@@ -401,7 +401,7 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     const int64_t nty = (a.OY + a.BY - 1) / a.BY;
     const int64_t blocks = nty * a.ntx;
     const size_t smem = size_t(2) * SR * a.W * sizeof(uint32_t);
-    void (*fn)(const MfmaArgs&, unsigned, size_t, hipStream_t) = nullptr;
+    void (*fn)(const MfmaArgs&, unsigned, size_t, LaunchContext&) = nullptr;
     const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
     const bool d1 = dx / a.gx == 1;
     const void* kp = x3 ? pick<true>(nkb, Q, d1, &fn) : pick<false>(nkb, Q, d1, &fn);
@@ -410,7 +410,7 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     // algorithmic work: 2 flops per output and tap (the FP32-equivalent count)
     ctx.kbegin(x3 ? "mfma_conv_bf16x3" : "mfma_conv_bf16",
                4.0 * double(p.x_numel + p.k_numel + p.y_numel), 2.0 * double(p.y_numel) * double(p.k_numel));
-    fn(a, unsigned(blocks), smem, ctx.stream());
+    fn(a, unsigned(blocks), smem, ctx);
     ctx.kend();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
