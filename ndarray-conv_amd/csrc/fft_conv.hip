@@ -708,8 +708,19 @@ struct ColsArgs {
     AxisSel outsel;                  // rows of axis d to write back
     int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse,
                                      // 3 fused with the kernel spectrum already transformed
+    int32_t nblocks;                 // grid size
     LinePlan<R> pl;
 };
+
+// Workgroups are dispatched round-robin over the eight XCDs, each with its own
+// L2.  Neighbouring column tiles share cache lines (a tile holds Ct adjacent
+// bins of every row), so the eight XCDs take consecutive ranges of tiles:
+// workgroup g, the (g/8)-th on XCD g%8, gets slot start(g%8) + g/8.  The map
+// is a permutation of [0, n) for every n.
+__device__ __forceinline__ int64_t xcd_slot(int64_t g, int64_t n) {
+    const int64_t per = n >> 3, rem = n & 7, x = g & 7, k = g >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+}
 
 template <typename R>
 __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
@@ -724,8 +735,9 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     C* b2 = b1 + PL;
     const int T = blockDim.x;
 
-    const int job = (a.mode == 0 && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
-    int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int64_t g = xcd_slot(blockIdx.x, a.nblocks);
+    const int job = (a.mode == 0 && g >= a.nblocks0) ? 1 : 0;
+    int64_t blk = job == 0 ? g : g - a.nblocks0;
     const int tile = int(blk % a.ntiles);
     int64_t rem = blk / a.ntiles;
     int64_t base = 0;
@@ -1067,6 +1079,7 @@ struct BigColArgs {
     int64_t nlines[2];
     int32_t nout;
     int32_t L1, L2, Ct, ntiles, nblocks0;
+    int32_t nblocks;              // grid size
     int32_t inv;                  // 0: forward from x / k (two jobs); 1: inverse to y (crop)
     AxisSel lastsel;              // crop along the long axis (inverse)
     R scale;
@@ -1083,8 +1096,9 @@ __global__ void __launch_bounds__(kMaxThreads) bigcol_kernel(BigColArgs<R, E> a)
     C* b0 = tw + a.pl.ntw;
     C* b1 = b0 + pp_len(L1) * Ct;
     const int T = blockDim.x;
-    const int job = (!a.inv && blockIdx.x >= unsigned(a.nblocks0)) ? 1 : 0;
-    const int64_t blk = job == 0 ? blockIdx.x : blockIdx.x - a.nblocks0;
+    const int64_t g = xcd_slot(blockIdx.x, a.nblocks);   // neighbouring tiles on one XCD
+    const int job = (!a.inv && g >= a.nblocks0) ? 1 : 0;
+    const int64_t blk = job == 0 ? g : g - a.nblocks0;
     const int tile = int(blk % a.ntiles);
     int64_t line = blk / a.ntiles;
     // outer coordinates of this line
@@ -1585,6 +1599,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const double lines = double(a.nlines[0] + a.nlines[1]);
         const double bytes = inv ? double(es) * lines * Fl + esz * double(p.y_numel)
                                  : esz * double(p.x_numel + p.k_numel) + double(es) * lines * Fl;
+        a.nblocks = int32_t(nb0 + nb1);
         ctx.kbegin(inv ? "fft_bigcol_inv" : "fft_bigcol_fwd", bytes, 5.0 * lines * Fl * std::log2(double(L1)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&bigcol_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
         NDC_LAUNCH(ctx, ctx.stream(), (bigcol_kernel<R, E>),
@@ -1703,6 +1718,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const int64_t nb0 = a.nlines_outer[0] * a.ntiles;
         const int64_t nb1 = cmode == 0 ? a.nlines_outer[1] * a.ntiles : 0;
         a.nblocks0 = int32_t(nb0);
+        a.nblocks = int32_t(nb0 + nb1);
         // compulsory traffic: the rows read (valid / all) and the rows written
         double lines0 = double(a.nlines_outer[0]) * H, lines1 = double(a.nlines_outer[1]) * H;
         double bytes = 0, nfft = 0;
@@ -1802,6 +1818,7 @@ struct LineFftArgs {
     int32_t Ct;          // lines per block: along inner (columns) or consecutive rows (inner == 1)
     int32_t inv;
     int32_t ntiles;      // column tiles per outer index
+    int32_t nblocks;     // grid size
     R scale;
     LinePlan<R> pl;      // complex length transformed: F/2 for even real lines, else n
 };
@@ -1821,8 +1838,9 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
     stage_table(tw, a.pl.tw, a.pl.ntw, T);
     if (a.inner > 1) {
         // columns: Ct lines at consecutive inner indices, element (c, j) at pp(j)*Ct + c
-        const int64_t o = blockIdx.x / a.ntiles;
-        const int64_t i0 = int64_t(blockIdx.x % a.ntiles) * Ct;
+        const int64_t g = xcd_slot(blockIdx.x, a.nblocks);   // neighbouring tiles on one XCD
+        const int64_t o = g / a.ntiles;
+        const int64_t i0 = int64_t(g % a.ntiles) * Ct;
         const int nc = int(imin64(Ct, a.inner - i0));
         const int64_t base = o * int64_t(a.n) * a.inner + i0;
         const float rC = 1.0f / float(Ct);
@@ -1997,6 +2015,7 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
         return NDC_ERR_UNSUPPORTED;
     }
     if (blocks <= 0) return NDC_OK;
+    a.nblocks = int32_t(blocks);
     const double bytes = double(outer) * double(inner) * double(n) * double(es) * 2.0;
     ctx.kbegin("fft_processor_lines", bytes, 5.0 * double(outer) * double(inner) * L * std::log2(double(std::max(2, L))));
     NDC_LAUNCH(ctx, ctx.stream(), (line_fft_kernel<R>),
