@@ -22,8 +22,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTileX = 64;           // outputs per tile row: one wavefront
-constexpr int kTileY = 16;           // tile rows
-constexpr int kItems = kTileY * kTileX / kThreads;
+// Tile rows: 32 (8 outputs per thread) for elements up to 8 bytes, 16 for
+// 16-byte elements, whose 8-output window would spill.
+template <typename T> constexpr int kTileY = sizeof(T) > 8 ? 16 : 32;
+template <typename T> constexpr int kItems = kTileY<T> * kTileX / kThreads;
 
 // Region cells staged per thread per round (fewer for 16-byte elements, which
 // would otherwise spill).
@@ -112,9 +114,10 @@ __device__ __forceinline__ bool plane_source(const DirectArgs<T>& a, int32_t cy,
 }
 
 // SLIDE (stride equal to dilation along the row, so consecutive outputs read
-// consecutive staged columns): each thread computes 4 adjacent outputs of
-// one tile row from a register window of kCX + 3 staged values per kCX
-// kernel columns, instead of one LDS read per product.  The taps of an
+// consecutive staged columns): each thread computes kItems<T> (8, or 4 for
+// 16-byte elements) adjacent outputs of one tile row from a register window of
+// kCX + kItems<T> - 1 staged values per kCX kernel columns, instead of one LDS
+// read per product.  The taps of an
 // output are summed in the same order in both forms.
 template <typename T, bool SLIDE>
 __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) {
@@ -142,22 +145,22 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
             ostr *= a.out[e];
         }
     }
-    const int64_t oy0 = int64_t(ty) * kTileY, ox0 = int64_t(tx) * kTileX;
+    const int64_t oy0 = int64_t(ty) * kTileY<T>, ox0 = int64_t(tx) * kTileX;
     // plain form: column lx, rows ly0 + 4i; sliding form: row ly0, columns lx + i
-    const int lx = SLIDE ? (threadIdx.x % (kTileX / kItems)) * kItems : threadIdx.x % kTileX;
-    const int ly0 = SLIDE ? threadIdx.x / (kTileX / kItems) : threadIdx.x / kTileX;
+    const int lx = SLIDE ? (threadIdx.x % (kTileX / kItems<T>)) * kItems<T> : threadIdx.x % kTileX;
+    const int ly0 = SLIDE ? threadIdx.x / (kTileX / kItems<T>) : threadIdx.x / kTileX;
 
-    T acc[kItems];
+    T acc[kItems<T>];
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::zero();
+    for (int i = 0; i < kItems<T>; ++i) acc[i] = Arith<T>::zero();
 
     const int64_t sy = a.s[ay], sx = a.s[ax], dy = a.d[ay], dx = a.d[ax];
     // region offsets: gcd(s, d) divides both terms, so the subsampled index is
     // ly*(s/g) + cy*(d/g)
     const int sy_ = int(sy / a.gy), dy_ = int(dy / a.gy), sx_ = int(sx / a.gx), dx_ = int(dx / a.gx);
-    int base[kItems];
+    int base[kItems<T>];
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) base[i] = (ly0 + i * (kThreads / kTileX)) * sy_ * a.RX + lx * sx_;
+    for (int i = 0; i < kItems<T>; ++i) base[i] = (ly0 + i * (kThreads / kTileX)) * sy_ * a.RX + lx * sx_;
 
     for (int64_t to = 0; to < a.ktaps_outer; ++to) {
         // outer tap -> padded coordinate per outer axis -> border code
@@ -237,11 +240,11 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                         const T* rrow = region + (ly0 * sy_ + cy * dy_) * a.RX + lx;
                         const T* trow = taps + cy * kxn;
                         for (int cx0 = 0; cx0 < kxn; cx0 += kCX) {
-                            // window values this chunk uses: up to column lx + kxn + 2 < RX
-                            const int nw = min(kCX, kxn - cx0) + kItems - 1;
-                            T w[kCX + kItems - 1];
+                            // window values this chunk uses: up to column lx + kxn + kItems<T> - 2 < RX
+                            const int nw = min(kCX, kxn - cx0) + kItems<T> - 1;
+                            T w[kCX + kItems<T> - 1];
 #pragma unroll
-                            for (int i = 0; i < kCX + kItems - 1; ++i)
+                            for (int i = 0; i < kCX + kItems<T> - 1; ++i)
                                 w[i] = i < nw ? rrow[cx0 + i] : Arith<T>::zero();
 #pragma unroll
                             for (int c = 0; c < kCX; ++c) {
@@ -249,7 +252,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                                 const T v = wave_uniform(trow[cx0 + c]);
                                 if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
 #pragma unroll
-                                for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
+                                for (int i = 0; i < kItems<T>; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
                             }
                         }
                     }
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                             if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
                             const int off = roff + cx * dx_;
 #pragma unroll
-                            for (int i = 0; i < kItems; ++i)
+                            for (int i = 0; i < kItems<T>; ++i)
                                 acc[i] = Arith<T>::mac(acc[i], region[base[i] + off], v);
                         }
                     }
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
         const int64_t oy = oy0 + ly0;
         if (oy < a.out[ay]) {
 #pragma unroll
-            for (int i = 0; i < kItems; ++i) {
+            for (int i = 0; i < kItems<T>; ++i) {
                 const int64_t ox = ox0 + lx + i;
                 if (ox < a.out[ax]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
             }
@@ -282,7 +285,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
         const int64_t ox = ox0 + lx;
         if (ox < a.out[ax]) {
 #pragma unroll
-            for (int i = 0; i < kItems; ++i) {
+            for (int i = 0; i < kItems<T>; ++i) {
                 const int64_t oy = oy0 + ly0 + i * (kThreads / kTileX);
                 if (oy < a.out[ay]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
             }
@@ -360,7 +363,7 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     // Choose the kernel chunk so that region + taps fit the LDS budget.
     const size_t budget = 48 * 1024;
     auto region_elems = [&](int64_t kyc, int64_t kxc) {
-        const int64_t ry = ((kTileY - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
+        const int64_t ry = ((kTileY<T> - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
         const int64_t rx = ((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1;
         return std::make_pair(ry, rx);
     };
@@ -384,7 +387,7 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     a.RY = int(rr.first);
     a.RX = int(rr.second);
     a.ntx = int((a.out[ax] + kTileX - 1) / kTileX);
-    a.nty = int((a.out[ay] + kTileY - 1) / kTileY);
+    a.nty = int((a.out[ay] + kTileY<T> - 1) / kTileY<T>);
     const int64_t nblocks = a.nouter * a.ntx * a.nty;
     if (nblocks <= 0) return NDC_OK;
     if (nblocks > (int64_t(1) << 31) - 1) {
