@@ -76,8 +76,29 @@ struct DirectArgs {
     int32_t KYc, KXc;   // kernel chunk staged per pass
     int32_t gy, gx;     // gcd(stride, dilation) per plane axis: region subsampling
     int32_t RY, RX;     // region extents in LDS
+    int32_t TO;         // element offset of the taps in LDS, after the region, 16-byte aligned
+    int32_t KS;         // tap row stride in LDS: KXc rounded up to kCX (zero-filled)
     int32_t ntx, nty;   // tiles per plane
 };
+
+// Elements per 16-byte LDS read of the sliding form's taps: 4 for 4-byte
+// elements; 1 (scalar reads) otherwise.
+template <typename T> constexpr int vec_elems() { return sizeof(T) == 4 ? 4 : 1; }
+
+// N 4-byte elements from 16-byte aligned LDS at p, in 16-byte reads.
+template <typename T, int N>
+__device__ __forceinline__ void lds_vec_load(const T* p, T (&out)[N]) {
+    static_assert(sizeof(T) == 4 && N % 4 == 0, "whole 16-byte reads of 4-byte elements");
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int j = 0; j < N / 4; ++j) {
+        const uint4 v = q[j];
+        out[4 * j + 0] = __builtin_bit_cast(T, v.x);
+        out[4 * j + 1] = __builtin_bit_cast(T, v.y);
+        out[4 * j + 2] = __builtin_bit_cast(T, v.z);
+        out[4 * j + 3] = __builtin_bit_cast(T, v.w);
+    }
+}
 
 // Border-map code of padded cell q of axis e (plan.cpp axis_map, position by
 // position): an index into x, or kUnwritten / kConstFront / kConstBack.
@@ -121,9 +142,9 @@ __device__ __forceinline__ bool plane_source(const DirectArgs<T>& a, int32_t cy,
 // output are summed in the same order in both forms.
 template <typename T, bool SLIDE>
 __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     T* region = reinterpret_cast<T*>(smem_raw);
-    T* taps = region + a.RY * a.RX;
+    T* taps = region + a.TO;
 
     const int ay = a.No - 2, ax = a.No - 1;
     int64_t bid = blockIdx.x;
@@ -226,30 +247,44 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                         if (i < nreg) region[i] = ok[u] ? raw[u] : fill[u];
                     }
                 }
-                // stage the kernel taps of this chunk (value of reversed-view tap t)
-                for (int i = threadIdx.x; i < kyn * kxn; i += kThreads) {
-                    const int cy = i / kxn, cx = i - cy * kxn;
-                    const int64_t ty_ = ky0 + cy, tx_ = kx0 + cx;
-                    const int64_t kiy = a.rev ? a.K[ay] - 1 - ty_ : ty_;
-                    const int64_t kix = a.rev ? a.K[ax] - 1 - tx_ : tx_;
-                    taps[i] = a.k[koff + kiy * a.kstr[ay] + kix * a.kstr[ax]];
+                // stage the kernel taps of this chunk (value of reversed-view tap
+                // t), rows KS apart, zeros past kxn
+                for (int i = threadIdx.x; i < kyn * a.KS; i += kThreads) {
+                    const int cy = i / a.KS, cx = i - cy * a.KS;
+                    T v = Arith<T>::zero();
+                    if (cx < kxn) {
+                        const int64_t ty_ = ky0 + cy, tx_ = kx0 + cx;
+                        const int64_t kiy = a.rev ? a.K[ay] - 1 - ty_ : ty_;
+                        const int64_t kix = a.rev ? a.K[ax] - 1 - tx_ : tx_;
+                        v = a.k[koff + kiy * a.kstr[ay] + kix * a.kstr[ax]];
+                    }
+                    taps[i] = v;
                 }
                 __syncthreads();
                 if constexpr (SLIDE) {
                     for (int cy = 0; cy < kyn; ++cy) {
                         const T* rrow = region + (ly0 * sy_ + cy * dy_) * a.RX + lx;
-                        const T* trow = taps + cy * kxn;
+                        const T* trow = taps + cy * a.KS;
                         for (int cx0 = 0; cx0 < kxn; cx0 += kCX) {
-                            // window values this chunk uses: up to column lx + kxn + kItems<T> - 2 < RX
+                            // window values this chunk uses: up to column lx + kxn + kItems - 2 < RX
                             const int nw = min(kCX, kxn - cx0) + kItems<T> - 1;
                             T w[kCX + kItems<T> - 1];
 #pragma unroll
                             for (int i = 0; i < kCX + kItems<T> - 1; ++i)
                                 w[i] = i < nw ? rrow[cx0 + i] : Arith<T>::zero();
+                            // the chunk's kCX taps (a broadcast read; trow + cx0 is 16-byte
+                            // aligned: TO, KS and cx0 are multiples of vec_elems)
+                            T tv[kCX];
+                            if constexpr (vec_elems<T>() > 1) {
+                                lds_vec_load(trow + cx0, tv);
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < kCX; ++c) tv[c] = trow[cx0 + c];
+                            }
 #pragma unroll
                             for (int c = 0; c < kCX; ++c) {
-                                if (cx0 + c >= kxn) break;
-                                const T v = wave_uniform(trow[cx0 + c]);
+                                // taps past kxn are the zero padding of the row: skipped below
+                                const T v = wave_uniform(tv[c]);
                                 if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
 #pragma unroll
                                 for (int i = 0; i < kItems<T>; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
@@ -260,7 +295,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                     for (int cy = 0; cy < kyn; ++cy) {
                         const int roff = cy * dy_ * a.RX;
                         for (int cx = 0; cx < kxn; ++cx) {
-                            const T v = taps[cy * kxn + cx];
+                            const T v = taps[cy * a.KS + cx];
                             if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
                             const int off = roff + cx * dx_;
 #pragma unroll
@@ -360,32 +395,43 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     a.gx = int(gcd64(a.s[ax], a.d[ax]));
     if (a.gy <= 0) a.gy = 1;
     if (a.gx <= 0) a.gx = 1;
-    // Choose the kernel chunk so that region + taps fit the LDS budget.
+    // Choose the kernel chunk so that region + taps fit the LDS budget.  LDS
+    // layout: the region (RY rows of RX, the staged width rounded up to odd),
+    // then the taps in rows of KS (a multiple of kCX, zero-filled), 16-byte
+    // aligned.
     const size_t budget = 48 * 1024;
-    auto region_elems = [&](int64_t kyc, int64_t kxc) {
-        const int64_t ry = ((kTileY<T> - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
-        const int64_t rx = ((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1;
-        return std::make_pair(ry, rx);
+    constexpr int64_t E = vec_elems<T>();
+    struct Layout { int64_t ry, rx, to, ks, total; };
+    auto layout = [&](int64_t kyc, int64_t kxc) {
+        Layout l;
+        l.ry = ((kTileY<T> - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
+        // odd row stride: a wave's eight tile rows start on distinct LDS banks
+        l.rx = (((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1) | 1;
+        l.to = (l.ry * l.rx + E - 1) / E * E;
+        l.ks = (kxc + kCX - 1) / kCX * kCX;
+        l.total = l.to + kyc * l.ks;
+        return l;
     };
     int64_t kyc = a.K[ay], kxc = a.K[ax];
     for (;;) {
-        auto rr = region_elems(kyc, kxc);
-        const size_t bytes = size_t(rr.first * rr.second + kyc * kxc) * sizeof(T);
+        const size_t bytes = size_t(layout(kyc, kxc).total) * sizeof(T);
         if (bytes <= budget || (kyc == 1 && kxc == 1)) break;
         if (kyc >= kxc && kyc > 1) kyc = (kyc + 1) / 2;
         else if (kxc > 1) kxc = (kxc + 1) / 2;
         else kyc = (kyc + 1) / 2;
     }
-    auto rr = region_elems(kyc, kxc);
-    const size_t smem = size_t(rr.first * rr.second + kyc * kxc) * sizeof(T);
+    const Layout lay = layout(kyc, kxc);
+    const size_t smem = size_t(lay.total) * sizeof(T);
     if (smem > ctx.max_lds()) {
         ctx.err = "direct convolution: stride/dilation too large for the LDS tile";
         return NDC_ERR_UNSUPPORTED;
     }
     a.KYc = int(kyc);
     a.KXc = int(kxc);
-    a.RY = int(rr.first);
-    a.RX = int(rr.second);
+    a.RY = int(lay.ry);
+    a.RX = int(lay.rx);
+    a.TO = int(lay.to);
+    a.KS = int(lay.ks);
     a.ntx = int((a.out[ax] + kTileX - 1) / kTileX);
     a.nty = int((a.out[ay] + kTileY<T> - 1) / kTileY<T>);
     const int64_t nblocks = a.nouter * a.ntx * a.nty;
