@@ -22,10 +22,11 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kTileX = 64;           // outputs per tile row: one wavefront
-// Tile rows: 32 (8 outputs per thread) for elements up to 8 bytes, 16 for
-// 16-byte elements, whose 8-output window would spill.
-template <typename T> constexpr int kTileY = sizeof(T) > 8 ? 16 : 32;
-template <typename T> constexpr int kItems = kTileY<T> * kTileX / kThreads;
+// Largest tile height (a kernel template argument TY; kTileX * TY / kThreads
+// outputs per thread): 64 for elements up to 4 bytes, 32 for 8-byte and 16 for
+// 16-byte elements, whose wider windows would spill.  4-byte launches use 32
+// rows when 64 would leave the chip underfilled or waste rows.
+template <typename T> constexpr int kMaxTileY = sizeof(T) > 8 ? 16 : (sizeof(T) > 4 ? 32 : 64);
 
 // Region cells staged per thread per round (fewer for 16-byte elements, which
 // would otherwise spill).
@@ -135,13 +136,14 @@ __device__ __forceinline__ bool plane_source(const DirectArgs<T>& a, int32_t cy,
 }
 
 // SLIDE (stride equal to dilation along the row, so consecutive outputs read
-// consecutive staged columns): each thread computes kItems<T> (8, or 4 for
-// 16-byte elements) adjacent outputs of one tile row from a register window of
-// kCX + kItems<T> - 1 staged values per kCX kernel columns, instead of one LDS
+// consecutive staged columns): each thread computes kItems (16, 8 or 4)
+// adjacent outputs of one tile row from a register window of
+// kCX + kItems - 1 staged values per kCX kernel columns, instead of one LDS
 // read per product.  The taps of an
 // output are summed in the same order in both forms.
-template <typename T, bool SLIDE>
+template <typename T, bool SLIDE, int TY>
 __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) {
+    constexpr int kItems = TY * kTileX / kThreads;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* region = reinterpret_cast<T*>(smem_raw);
     T* taps = region + a.TO;
@@ -166,22 +168,22 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
             ostr *= a.out[e];
         }
     }
-    const int64_t oy0 = int64_t(ty) * kTileY<T>, ox0 = int64_t(tx) * kTileX;
+    const int64_t oy0 = int64_t(ty) * TY, ox0 = int64_t(tx) * kTileX;
     // plain form: column lx, rows ly0 + 4i; sliding form: row ly0, columns lx + i
-    const int lx = SLIDE ? (threadIdx.x % (kTileX / kItems<T>)) * kItems<T> : threadIdx.x % kTileX;
-    const int ly0 = SLIDE ? threadIdx.x / (kTileX / kItems<T>) : threadIdx.x / kTileX;
+    const int lx = SLIDE ? (threadIdx.x % (kTileX / kItems)) * kItems : threadIdx.x % kTileX;
+    const int ly0 = SLIDE ? threadIdx.x / (kTileX / kItems) : threadIdx.x / kTileX;
 
-    T acc[kItems<T>];
+    T acc[kItems];
 #pragma unroll
-    for (int i = 0; i < kItems<T>; ++i) acc[i] = Arith<T>::zero();
+    for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::zero();
 
     const int64_t sy = a.s[ay], sx = a.s[ax], dy = a.d[ay], dx = a.d[ax];
     // region offsets: gcd(s, d) divides both terms, so the subsampled index is
     // ly*(s/g) + cy*(d/g)
     const int sy_ = int(sy / a.gy), dy_ = int(dy / a.gy), sx_ = int(sx / a.gx), dx_ = int(dx / a.gx);
-    int base[kItems<T>];
+    int base[kItems];
 #pragma unroll
-    for (int i = 0; i < kItems<T>; ++i) base[i] = (ly0 + i * (kThreads / kTileX)) * sy_ * a.RX + lx * sx_;
+    for (int i = 0; i < kItems; ++i) base[i] = (ly0 + i * (kThreads / kTileX)) * sy_ * a.RX + lx * sx_;
 
     for (int64_t to = 0; to < a.ktaps_outer; ++to) {
         // outer tap -> padded coordinate per outer axis -> border code
@@ -267,10 +269,10 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                         const T* trow = taps + cy * a.KS;
                         for (int cx0 = 0; cx0 < kxn; cx0 += kCX) {
                             // window values this chunk uses: up to column lx + kxn + kItems - 2 < RX
-                            const int nw = min(kCX, kxn - cx0) + kItems<T> - 1;
-                            T w[kCX + kItems<T> - 1];
+                            const int nw = min(kCX, kxn - cx0) + kItems - 1;
+                            T w[kCX + kItems - 1];
 #pragma unroll
-                            for (int i = 0; i < kCX + kItems<T> - 1; ++i)
+                            for (int i = 0; i < kCX + kItems - 1; ++i)
                                 w[i] = i < nw ? rrow[cx0 + i] : Arith<T>::zero();
                             // the chunk's kCX taps (a broadcast read; trow + cx0 is 16-byte
                             // aligned: TO, KS and cx0 are multiples of vec_elems)
@@ -287,7 +289,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                                 const T v = wave_uniform(tv[c]);
                                 if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
 #pragma unroll
-                                for (int i = 0; i < kItems<T>; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
+                                for (int i = 0; i < kItems; ++i) acc[i] = Arith<T>::mac(acc[i], w[i + c], v);
                             }
                         }
                     }
@@ -299,7 +301,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                             if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
                             const int off = roff + cx * dx_;
 #pragma unroll
-                            for (int i = 0; i < kItems<T>; ++i)
+                            for (int i = 0; i < kItems; ++i)
                                 acc[i] = Arith<T>::mac(acc[i], region[base[i] + off], v);
                         }
                     }
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
         const int64_t oy = oy0 + ly0;
         if (oy < a.out[ay]) {
 #pragma unroll
-            for (int i = 0; i < kItems<T>; ++i) {
+            for (int i = 0; i < kItems; ++i) {
                 const int64_t ox = ox0 + lx + i;
                 if (ox < a.out[ax]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
             }
@@ -320,7 +322,7 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
         const int64_t ox = ox0 + lx;
         if (ox < a.out[ax]) {
 #pragma unroll
-            for (int i = 0; i < kItems<T>; ++i) {
+            for (int i = 0; i < kItems; ++i) {
                 const int64_t oy = oy0 + ly0 + i * (kThreads / kTileX);
                 if (oy < a.out[ay]) a.y[yoff + oy * a.out[ax] + ox] = acc[i];
             }
@@ -335,6 +337,21 @@ int64_t gcd64(int64_t a, int64_t b) {
         b = t;
     }
     return a;
+}
+
+// Launches the TY-row form; false when its LDS request does not fit.
+template <typename T, int TY>
+bool launch_tile(LaunchContext& ctx, bool slide, const DirectArgs<T>& a, int64_t nblocks, size_t smem) {
+    const void* kp = slide ? reinterpret_cast<const void*>(&direct_conv_kernel<T, true, TY>)
+                           : reinterpret_cast<const void*>(&direct_conv_kernel<T, false, TY>);
+    if (!lds_fits(ctx, kp, smem)) return false;
+    if (slide)
+        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, true, TY>),
+            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
+    else
+        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, false, TY>),
+            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
+    return true;
 }
 
 template <typename T>
@@ -395,6 +412,13 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     a.gx = int(gcd64(a.s[ax], a.d[ax]));
     if (a.gy <= 0) a.gy = 1;
     if (a.gx <= 0) a.gx = 1;
+    // tile height: the largest unless, for 4-byte elements, 64-row tiles would
+    // give under four workgroups per CU or waste over a tenth of the rows
+    int ty = kMaxTileY<T>;
+    if (ty == 64) {
+        const int64_t ntx = (a.out[ax] + kTileX - 1) / kTileX, nty = (a.out[ay] + 63) / 64;
+        if (a.nouter * ntx * nty < 4 * int64_t(ctx.cu_count()) || (nty * 64 - a.out[ay]) * 10 > a.out[ay]) ty = 32;
+    }
     // Choose the kernel chunk so that region + taps fit the LDS budget.  LDS
     // layout: the region (RY rows of RX, the staged width rounded up to odd),
     // then the taps in rows of KS (a multiple of kCX, zero-filled), 16-byte
@@ -404,7 +428,7 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     struct Layout { int64_t ry, rx, to, ks, total; };
     auto layout = [&](int64_t kyc, int64_t kxc) {
         Layout l;
-        l.ry = ((kTileY<T> - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
+        l.ry = ((ty - 1) * a.s[ay] + (kyc - 1) * a.d[ay]) / a.gy + 1;
         // odd row stride: a wave's eight tile rows start on distinct LDS banks
         l.rx = (((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1) | 1;
         l.to = (l.ry * l.rx + E - 1) / E * E;
@@ -433,7 +457,7 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
     a.TO = int(lay.to);
     a.KS = int(lay.ks);
     a.ntx = int((a.out[ax] + kTileX - 1) / kTileX);
-    a.nty = int((a.out[ay] + kTileY<T> - 1) / kTileY<T>);
+    a.nty = int((a.out[ay] + ty - 1) / ty);
     const int64_t nblocks = a.nouter * a.ntx * a.nty;
     if (nblocks <= 0) return NDC_OK;
     if (nblocks > (int64_t(1) << 31) - 1) {
@@ -444,15 +468,11 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
                2.0 * double(p.y_numel) * double(p.k_numel));
     // the sliding form needs consecutive outputs on consecutive staged columns
     const bool slide = a.s[ax] == a.d[ax];
-    const void* kp = slide ? reinterpret_cast<const void*>(&direct_conv_kernel<T, true>)
-                           : reinterpret_cast<const void*>(&direct_conv_kernel<T, false>);
-    if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
-    if (slide)
-        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, true>),
-            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
-    else
-        NDC_LAUNCH(ctx, ctx.stream(), (direct_conv_kernel<T, false>),
-            dim3(unsigned(nblocks)), dim3(kThreads), smem, a);
+    if (ty == kMaxTileY<T>) {
+        if (!launch_tile<T, kMaxTileY<T>>(ctx, slide, a, nblocks, smem)) return NDC_ERR_UNSUPPORTED;
+    } else if constexpr (kMaxTileY<T> == 64) {
+        if (!launch_tile<T, 32>(ctx, slide, a, nblocks, smem)) return NDC_ERR_UNSUPPORTED;
+    }
     ctx.kend();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
