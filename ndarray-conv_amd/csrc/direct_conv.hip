@@ -32,8 +32,9 @@ template <typename T> constexpr int kMaxTileY = sizeof(T) > 8 ? 16 : (sizeof(T) 
 // would otherwise spill).
 template <typename T> constexpr int staged_per_round() { return sizeof(T) > 8 ? 2 : 8; }
 
-// Kernel columns per register window of the sliding form.
-constexpr int kCX = 16;
+// Kernel columns per register window of the sliding form (32 for 4-byte
+// elements, 16 for wider ones, whose windows would spill).
+template <typename T> constexpr int kCX = sizeof(T) == 4 ? 32 : 16;
 
 // A value every lane holds identically (a broadcast LDS read), moved to
 // scalar registers: the zero-tap test becomes a scalar branch and the
@@ -267,24 +268,24 @@ __global__ void __launch_bounds__(kThreads) direct_conv_kernel(DirectArgs<T> a) 
                     for (int cy = 0; cy < kyn; ++cy) {
                         const T* rrow = region + (ly0 * sy_ + cy * dy_) * a.RX + lx;
                         const T* trow = taps + cy * a.KS;
-                        for (int cx0 = 0; cx0 < kxn; cx0 += kCX) {
+                        for (int cx0 = 0; cx0 < kxn; cx0 += kCX<T>) {
                             // window values this chunk uses: up to column lx + kxn + kItems - 2 < RX
-                            const int nw = min(kCX, kxn - cx0) + kItems - 1;
-                            T w[kCX + kItems - 1];
+                            const int nw = min(kCX<T>, kxn - cx0) + kItems - 1;
+                            T w[kCX<T> + kItems - 1];
 #pragma unroll
-                            for (int i = 0; i < kCX + kItems - 1; ++i)
+                            for (int i = 0; i < kCX<T> + kItems - 1; ++i)
                                 w[i] = i < nw ? rrow[cx0 + i] : Arith<T>::zero();
                             // the chunk's kCX taps (a broadcast read; trow + cx0 is 16-byte
                             // aligned: TO, KS and cx0 are multiples of vec_elems)
-                            T tv[kCX];
+                            T tv[kCX<T>];
                             if constexpr (vec_elems<T>() > 1) {
                                 lds_vec_load(trow + cx0, tv);
                             } else {
 #pragma unroll
-                                for (int c = 0; c < kCX; ++c) tv[c] = trow[cx0 + c];
+                                for (int c = 0; c < kCX<T>; ++c) tv[c] = trow[cx0 + c];
                             }
 #pragma unroll
-                            for (int c = 0; c < kCX; ++c) {
+                            for (int c = 0; c < kCX<T>; ++c) {
                                 // taps past kxn are the zero padding of the row: skipped below
                                 const T v = wave_uniform(tv[c]);
                                 if (Arith<T>::is_zero(v)) continue;   // zero taps are not in the offset list
@@ -432,7 +433,7 @@ ndc_status launch_typed(const Plan& p, const void* x, const void* k, void* y, La
         // odd row stride: a wave's eight tile rows start on distinct LDS banks
         l.rx = (((kTileX - 1) * a.s[ax] + (kxc - 1) * a.d[ax]) / a.gx + 1) | 1;
         l.to = (l.ry * l.rx + E - 1) / E * E;
-        l.ks = (kxc + kCX - 1) / kCX * kCX;
+        l.ks = (kxc + kCX<T> - 1) / kCX<T> * kCX<T>;
         l.total = l.to + kyc * l.ks;
         return l;
     };
