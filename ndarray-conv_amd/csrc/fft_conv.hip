@@ -561,6 +561,25 @@ __global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E>
     const int nl = int(imin64(a.lpb, a.nlines[job] - line0));
     const int ax = a.nout;   // last axis index
 
+    // post-processing factors of this thread's first store round, loaded now
+    // so that they arrive with the input rather than after the transform
+    // (single precision: the double kernels have no registers to spare)
+    constexpr bool kPre = sizeof(R) == 4;
+    const int H = a.H;
+    const int nH = nl * H;
+    const float rH = 1.0f / float(H);
+    const C* wpost = a.pl.tw + a.pl.post;
+    C wk0[kU];
+    if (kPre && a.mode == 0) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int it = int(threadIdx.x) + u * T;
+            const int itc = it < nH ? it : 0;
+            const int b = fdiv(itc, H, rH), k = itc - b * H;
+            wk0[u] = wpost[k];
+        }
+    }
+
     // per-line outer resolution (closed-form maps: no memory reads)
     if (threadIdx.x < nl) {
         int rem = line0 + int(threadIdx.x);
@@ -656,13 +675,12 @@ __global__ void __launch_bounds__(kMaxThreads) rows_fwd_kernel(RowsFwdArgs<R, E>
     // store H bins per row; the post-processing factors of a round are loaded
     // together before any of them is used
     Cx<R>* out = a.spec[job];
-    const int H = a.H;
-    const int nH = nl * H;
-    const float rH = 1.0f / float(H);
-    const C* wpost = a.pl.tw + a.pl.post;
     for (int base = threadIdx.x; base < nH; base += kU * T) {
         C wk[kU];
-        if (a.mode == 0) {
+        if (kPre && a.mode == 0 && base == int(threadIdx.x)) {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) wk[u] = wk0[u];
+        } else if (a.mode == 0) {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int it = base + u * T;
