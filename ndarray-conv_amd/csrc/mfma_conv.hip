@@ -89,7 +89,11 @@ __device__ __forceinline__ bool pad_source(const MfmaArgs& a, int32_t cy, int32_
 template <int NKB, int Q, bool X3, bool D1>
 __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     constexpr int kSR = rows_per_batch<Q>();
-    extern __shared__ uint32_t rows_lds[];   // [2][kSR][W] packed (hi, lo)
+    // D1: per buffer four BF16 planes of kSR x W (W even): hi, lo, and both
+    // shifted by one element, so a K-block's 8 consecutive columns are 4
+    // aligned 32-bit reads in operand order from one of them; otherwise
+    // [2][kSR][W] 32-bit entries packing (hi, lo)
+    extern __shared__ uint32_t rows_lds[];
     __shared__ int32_t s_row[kMaxChain];     // border code of chain row t
     __shared__ int32_t s_col[kMaxW];         // border code of staged column j
     const int lane = threadIdx.x & 63;
@@ -189,13 +193,30 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
 #pragma unroll
                 for (int i = 0; i < kPer; ++i) raw[i] = a.x[off[i]];   // offset 0 (a valid element) where not used
             };
+            const int plane = kSR * a.W;   // entries of one BF16 plane (D1)
             auto store = [&](uint32_t* dst) {
+                uint16_t* h16 = reinterpret_cast<uint16_t*>(dst);
 #pragma unroll
                 for (int i = 0; i < kPer; ++i) {
                     const int e = threadIdx.x + i * kThreads;
-                    if (e < kSR * a.W) dst[e] = split_pack((okm >> i & 1u) ? raw[i] : fill[i]);
+                    if (e < kSR * a.W) {
+                        const uint32_t v = split_pack((okm >> i & 1u) ? raw[i] : fill[i]);
+                        if constexpr (D1) {
+                            // hi at plane 0, lo at plane 1; the shifted planes 2, 3
+                            // hold entry e at e - 1
+                            h16[e] = uint16_t(v);
+                            if (e > 0) h16[2 * plane + e - 1] = uint16_t(v);
+                            if (X3) {
+                                h16[plane + e] = uint16_t(v >> 16);
+                                if (e > 0) h16[3 * plane + e - 1] = uint16_t(v >> 16);
+                            }
+                        } else {
+                            dst[e] = v;
+                        }
+                    }
                 }
             };
+            const int bufw = D1 ? 2 * plane : plane;   // 32-bit words per buffer
             // host keeps W <= kMaxW, so kSR * W <= kPer * kThreads
             fetch(0);
             store(rows_lds);
@@ -203,13 +224,14 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
             int t = 0;
             for (int b = 0; b < nbatch; ++b) {
                 if (b + 1 < nbatch) fetch(b + 1);
-                const uint32_t* buf = rows_lds + (b & 1) * (kSR * a.W);
+                const uint32_t* buf = rows_lds + (b & 1) * bufw;
                 const int tend = min(nsteps, (b + 1) * kSR);
                 for (; t < tend;) {
 #pragma unroll
                     for (int s = 0; s < Q; ++s, ++t) {
                         if (t >= tend) break;
-                        const uint32_t* row = buf + (t - b * kSR) * a.W;
+                        const int rowoff = (t - b * kSR) * a.W;
+                        const uint32_t* row = buf + rowoff;
                         // Hankel A operands from the staged row
                         u32x4 ah[NKB], al[NKB];
 #pragma unroll
@@ -218,13 +240,29 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                             // every staged column of the K-block exists (W covers
                             // 16 * NKB kernel columns): the reads are unconditional
                             // and only a partial last block selects
-                            uint32_t p[8];
+                            if constexpr (D1) {
+                                // columns base..base+7 of the hi / lo plane, from the
+                                // shifted planes when base is odd: even half-word
+                                // offsets, so 32-bit reads of (column 2i, 2i+1) pairs
+                                const int odd = base & 1;
+                                const uint16_t* b16 = reinterpret_cast<const uint16_t*>(buf);
+                                const int oh = (odd ? 2 * plane : 0) + rowoff + base - odd;
+                                const uint32_t* ph = reinterpret_cast<const uint32_t*>(b16 + oh);
+                                const uint32_t* pl = reinterpret_cast<const uint32_t*>(b16 + oh + plane);
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
+                                for (int i = 0; i < 4; ++i) {
+                                    ah[kb][i] = ph[i];
+                                    al[kb][i] = X3 ? pl[i] : 0u;
+                                }
+                            } else {
+                                uint32_t p[8];
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
-                                al[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                                for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+                                    al[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                                }
                             }
                             if (16 * kb + 16 > KCg) {
 #pragma unroll
@@ -280,7 +318,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                         }
                     }
                 }
-                if (b + 1 < nbatch) store(rows_lds + ((b + 1) & 1) * (kSR * a.W));
+                if (b + 1 < nbatch) store(rows_lds + ((b + 1) & 1) * bufw);
                 __syncthreads();
             }
         }
@@ -384,6 +422,8 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     }
     nkb = std::min(nkb, int((a.KW + 15) / 16));
     a.W = int32_t(((kXB - 1) * sx + (16 * nkb - 1) * dx) / a.gx + 1);
+    const bool d1 = dx / a.gx == 1;
+    if (d1) a.W += a.W & 1;   // the BF16 planes keep rows 4-byte aligned (W <= kMaxW stays: kMaxW is even)
     a.KC = std::min<int32_t>(a.KW, 16 * nkb);
     a.npass_c = (a.KW + a.KC - 1) / a.KC;
     a.npass_r = (a.KH + 31) / 32;
@@ -400,10 +440,11 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     a.BY = int32_t(by);
     const int64_t nty = (a.OY + a.BY - 1) / a.BY;
     const int64_t blocks = nty * a.ntx;
-    const size_t smem = size_t(2) * SR * a.W * sizeof(uint32_t);
+    // two buffers of SR x W staged entries: four 2-byte BF16 planes per entry
+    // (D1) or one 32-bit (hi, lo) word
+    const size_t smem = size_t(2) * SR * a.W * (d1 ? 4 * sizeof(uint16_t) : sizeof(uint32_t));
     void (*fn)(const MfmaArgs&, unsigned, size_t, LaunchContext&) = nullptr;
     const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
-    const bool d1 = dx / a.gx == 1;
     const void* kp = x3 ? pick<true>(nkb, Q, d1, &fn) : pick<false>(nkb, Q, d1, &fn);
     if (!kp || !fn) return NDC_ERR_UNSUPPORTED;
     if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;
