@@ -727,6 +727,7 @@ struct ColsArgs {
     int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse,
                                      // 3 fused with the kernel spectrum already transformed
     int32_t nblocks;                 // grid size
+    int32_t joint;                   // mode 1: kernel and data columns transformed as one batch
     LinePlan<R> pl;
 };
 
@@ -770,7 +771,9 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     const int64_t sd = a.sstr[a.d];
     const float rC = 1.0f / float(Ct);
 
-    auto load = [&](const C* src, C* dstbuf, int valid) {
+    // tile column c of row t goes to dstbuf[pp(t) * es + c] (es = Ct, or 2 Ct
+    // in the fused mode's joint kernel + data buffer)
+    auto load = [&](const C* src, C* dstbuf, int valid, int es) {
         const int total = L * Ct;
         for (int b = threadIdx.x; b < total; b += kU * T) {
             C v[kU];
@@ -780,7 +783,7 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
                 const int it = b + u * T;
                 v[u] = cmake<C>(R(0), R(0));
                 const int t = fdiv(it, Ct, rC), c = it - t * Ct;
-                at[u] = pp(t) * Ct + c;
+                at[u] = pp(t) * es + c;
                 if (it < total && t < valid && c < nc) v[u] = src[base + int64_t(t) * sd + c0 + c];
             }
 #pragma unroll
@@ -793,14 +796,15 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
 
     stage_table(tw, a.pl.tw, a.pl.ntw, T);
     C* res;
+    int res_es = Ct;   // element stride of the result lines
     if (a.mode == 0) {
-        load(a.spec[job], b0, a.valid[job]);
+        load(a.spec[job], b0, a.valid[job], Ct);
         __syncthreads();
         res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, false, T) ? b1 : b0;
-    } else if (a.mode == 1) {
+    } else if (a.mode == 1 && !a.joint) {
         // kernel and data columns loaded together, then transformed in turn
-        load(a.spec[1], b1, a.valid[1]);
-        load(a.spec[0], b0, a.valid[0]);
+        load(a.spec[1], b1, a.valid[1], Ct);
+        load(a.spec[0], b0, a.valid[0], Ct);
         __syncthreads();
         const int ck = fft_lines<R, true>(b1, b2, a.pl, tw, Ct, 1, Ct, false, T);
         C* kres = ck ? b2 : b1;
@@ -811,11 +815,32 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         for (int it = threadIdx.x; it < PL; it += T) dres[it] = cmul(dres[it], kres[it]);
         __syncthreads();
         res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true, T) ? dfree : dres;
+    } else if (a.mode == 1) {
+        // kernel columns (lines 0..Ct-1) and data columns (lines Ct..2Ct-1) in
+        // one interleaved buffer, transformed together (twice the butterflies
+        // per stage, one set of stage barriers); the product lands on the data
+        // lines, which alone are inverted
+        C* j0 = b0;
+        C* j1 = b0 + 2 * PL;
+        load(a.spec[1], j0, a.valid[1], 2 * Ct);
+        load(a.spec[0], j0 + Ct, a.valid[0], 2 * Ct);
+        __syncthreads();
+        const int cj = fft_lines<R, true>(j0, j1, a.pl, tw, 2 * Ct, 1, 2 * Ct, false, T);
+        C* jr = cj ? j1 : j0;
+        C* jf = cj ? j0 : j1;
+        for (int it = threadIdx.x; it < PL; it += T) {
+            const int t = fdiv(it, Ct, rC), c = it - t * Ct;
+            const int e = t * 2 * Ct + c;   // it walks the LDS slots pp(t) of the tile (PL = pp_len(L) * Ct)
+            jr[e + Ct] = cmul(jr[e + Ct], jr[e]);
+        }
+        __syncthreads();
+        res = (fft_lines<R, true>(jr + Ct, jf + Ct, a.pl, tw, Ct, 1, 2 * Ct, true, T) ? jf : jr) + Ct;
+        res_es = 2 * Ct;
     } else if (a.mode == 3) {
         // kernel columns already transformed (kept spectrum): data forward,
         // product, inverse
-        load(a.spec[1], b1, L);
-        load(a.spec[0], b0, a.valid[0]);
+        load(a.spec[1], b1, L, Ct);
+        load(a.spec[0], b0, a.valid[0], Ct);
         __syncthreads();
         const int cd = fft_lines<R, true>(b0, b2, a.pl, tw, Ct, 1, Ct, false, T);
         C* dres = cd ? b2 : b0;
@@ -824,7 +849,7 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         __syncthreads();
         res = fft_lines<R, true>(dres, dfree, a.pl, tw, Ct, 1, Ct, true, T) ? dfree : dres;
     } else {
-        load(a.spec[0], b0, L);
+        load(a.spec[0], b0, L, Ct);
         __syncthreads();
         res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, true, T) ? b1 : b0;
     }
@@ -835,7 +860,7 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
         const int i = fdiv(it, Ct, rC), c = it - i * Ct;
         if (c >= nc) continue;
         const int t = int(a.outsel.start + int64_t(i) * a.outsel.step);
-        dst[base + int64_t(t) * sd + c0 + c] = res[pp(t) * Ct + c];
+        dst[base + int64_t(t) * sd + c0 + c] = res[pp(t) * res_es + c];
     }
 }
 
@@ -1718,10 +1743,14 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         else a.outsel = AxisSel{p.kdil[d] - 1, p.stride[d], p.out[d]};
         if (cmode == 0 && !data_job) a.nlines_outer[0] = 0;
         if (cmode == 0 && !kernel_job) a.nlines_outer[1] = 0;
-        const int nbuf = (cmode == 1 || cmode == 3) ? 3 : 2;
+        // fused: two joint kernel + data buffers (4 tiles) when they fit the
+        // budget below at 8 columns, else kernel, data and a spare (3 tiles)
+        auto smem_n = [&](int nb, int ct) { return (size_t(a.pl.ntw) + size_t(nb) * pp_len(Ld) * ct) * es; };
+        a.joint = cmode == 1 && smem_n(4, 8) <= std::min<size_t>(lds, 96 * 1024) ? 1 : 0;
+        const int nbuf = cmode == 1 ? (a.joint ? 4 : 3) : (cmode == 3 ? 3 : 2);
         // tile width: enough lines for coalescing, enough blocks to fill the chip
         int Ct = 16;
-        auto smem_of = [&](int ct) { return (size_t(a.pl.ntw) + size_t(nbuf) * pp_len(Ld) * ct) * es; };
+        auto smem_of = [&](int ct) { return smem_n(nbuf, ct); };
         while (Ct > 1 && smem_of(Ct) > std::min<size_t>(lds, 96 * 1024)) Ct /= 2;
         // 8 columns = 64-byte segments per row (f32): below that, partial-line
         // reads and writes cost more than the extra blocks gain
