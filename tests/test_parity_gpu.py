@@ -185,6 +185,22 @@ def test_conv_fft_long_last_axis(gpu, shape, kshape):
         check_conv_fft(x, k, ("custom", [1] * len(shape), [2] * len(shape)), "replicate", 2)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64, np.float64, np.complex128])
+@pytest.mark.parametrize("rows", [60, 250, 500])
+def test_conv_fft_fused_column_forms(gpu, dtype, rows):
+    # The fused axis-0 column pass transforms the kernel and data columns of a
+    # tile as one interleaved batch while four 8-column tiles fit 96 KB of LDS
+    # (axis-0 FFT lengths up to 384 for f32/c64, 192 for f64/c128), and one
+    # after the other beyond: 60 rows take the first form for every dtype,
+    # 500 the second, 250 the first for f32/c64 and the second for f64/c128.
+    rng = np.random.default_rng(rows + np.dtype(dtype).num)
+    x = rand_array(rng, [rows, 23], dtype)
+    k = rand_array(rng, [7, 4], dtype)
+    check_conv_fft(x, k, "same", ("custom", ["reflect", "circular"]), 2)
+    check_conv_fft(x, k, "full", "zeros")
+    check_conv_fft(x, k, ("custom", [3, 1], [2, 1]), "replicate", 1, False)
+
+
 def test_strided_views(gpu):
     rng = np.random.default_rng(5)
     base = rng.standard_normal((20, 30)).astype(np.float32)
