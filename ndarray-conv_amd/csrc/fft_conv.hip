@@ -486,10 +486,10 @@ __device__ __forceinline__ Cx<R> row_sample(int mode, int n, F&& val) {
 // LDS line in the padded layout)
 // (X = E + W_F^k O with E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2).
 template <typename C>
-__device__ __forceinline__ C r2c_bin(const C* Z, int M, int k, const C* wpost) {
+__device__ __forceinline__ C r2c_bin(const C* Z, int M, int k, const C* wpost, int es = 1) {
     using R = decltype(C{}.x);
-    const C zk = Z[pp(k == M ? 0 : k)];
-    const C zm = cconj(Z[pp(k == 0 ? 0 : M - k)]);
+    const C zk = Z[pp(k == M ? 0 : k) * es];
+    const C zm = cconj(Z[pp(k == 0 ? 0 : M - k) * es]);
     const C e = cscale(cadd(zk, zm), R(0.5));
     const C o = cmul_negi(cscale(csub(zk, zm), R(0.5)));
     return cadd(e, cmul(o, wpost[k]));
@@ -1004,6 +1004,7 @@ struct Fused1dArgs {
     const Cx<R>* ks;   // kernel spectrum bins computed by a previous pass, or null
     E* y;
     int32_t F, H, mode;
+    int32_t joint;     // no ks: kernel and data lines transformed as one batch (four line buffers)
     AxisSel lastsel;
     R scale;
     LinePlan<R> pl;
@@ -1020,7 +1021,7 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
     C* const buf2 = buf1 + pp_len(L);
     const int T = blockDim.x;
     const RowRes<E> rr{0, -1, zero_of<E>(), false, false};
-    auto load = [&](C* dst, bool kernel) {
+    auto load = [&](C* dst, bool kernel, int es) {
         for (int base = threadIdx.x; base < L; base += kU * T) {
             C z[kU];
 #pragma unroll
@@ -1035,28 +1036,52 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 const int n = base + u * T;
-                if (n < L) dst[pp(n)] = z[u];
+                if (n < L) dst[pp(n) * es] = z[u];
             }
         }
     };
     stage_table(tw, a.pl.tw, a.pl.ntw, T);
-    C* K = nullptr;
-    C* f0 = buf0;
-    C* f1 = buf1;
-    if (!a.ks) {
-        load(buf2, true);
-        load(buf0, false);
+    C* K = nullptr;   // kernel spectrum (LDS), element stride des
+    C* D;             // data spectrum, element stride des
+    C* W;             // free line for the product
+    C* other;         // W's partner in the inverse
+    int des = 1;
+    if (!a.ks && a.joint) {
+        // kernel (line 0) and data (line 1) interleaved in one buffer and
+        // transformed as one batch: one set of stage barriers and twice the
+        // butterflies per stage
+        C* j0 = buf0;
+        C* j1 = buf0 + 2 * pp_len(L);
+        load(j0, true, 2);
+        load(j0 + 1, false, 2);
         __syncthreads();
-        const int ck = fft_lines<R, false>(buf2, buf1, a.pl, tw, 1, pp_len(L), 1, false, T);
-        K = ck ? buf1 : buf2;
-        f1 = ck ? buf2 : buf1;
+        const int cj = fft_lines<R, true>(j0, j1, a.pl, tw, 2, 1, 2, false, T);
+        C* jr = cj ? j1 : j0;
+        C* jf = cj ? j0 : j1;
+        K = jr;
+        D = jr + 1;
+        des = 2;
+        W = jf;
+        other = jf + pp_len(L);
     } else {
-        load(buf0, false);
-        __syncthreads();
+        C* f0 = buf0;
+        C* f1 = buf1;
+        if (!a.ks) {
+            load(buf2, true, 1);
+            load(buf0, false, 1);
+            __syncthreads();
+            const int ck = fft_lines<R, false>(buf2, buf1, a.pl, tw, 1, pp_len(L), 1, false, T);
+            K = ck ? buf1 : buf2;
+            f1 = ck ? buf2 : buf1;
+        } else {
+            load(buf0, false, 1);
+            __syncthreads();
+        }
+        const int cd = fft_lines<R, false>(f0, f1, a.pl, tw, 1, pp_len(L), 1, false, T);
+        D = cd ? f1 : f0;
+        W = cd ? f0 : f1;   // free buffer
+        other = D;          // free once the product is formed
     }
-    const int cd = fft_lines<R, false>(f0, f1, a.pl, tw, 1, pp_len(L), 1, false, T);
-    C* D = cd ? f1 : f0;
-    C* W = cd ? f0 : f1;   // free buffer
     const C* wpost = tw + a.pl.post;
     if (a.mode == 0) {
         // bins k and M-k together: forward post-process both spectra,
@@ -1064,8 +1089,8 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
         const int M = L;
         for (int k = threadIdx.x; k <= M / 2; k += T) {
             const int km = M - k;
-            C Pk = cmul(r2c_bin(D, M, k, wpost), a.ks ? a.ks[k] : r2c_bin(K, M, k, wpost));
-            C Pm = cmul(r2c_bin(D, M, km, wpost), a.ks ? a.ks[km] : r2c_bin(K, M, km, wpost));
+            C Pk = cmul(r2c_bin(D, M, k, wpost, des), a.ks ? a.ks[k] : r2c_bin(K, M, k, wpost, des));
+            C Pm = cmul(r2c_bin(D, M, km, wpost, des), a.ks ? a.ks[km] : r2c_bin(K, M, km, wpost, des));
             if (k == 0) { Pk.y = 0; Pm.y = 0; }   // Im of DC and Nyquist ignored by c2r
             W[pp(k == M ? 0 : k)] = c2r_pre(Pk, cconj(Pm), k, wpost);
             if (km != k && km < M) W[pp(km)] = c2r_pre(Pm, cconj(Pk), km, wpost);
@@ -1077,7 +1102,7 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
             for (int k = threadIdx.x; k < L; k += T)
                 W[pp(k)] = cmul(D[pp(k)], k < a.H ? a.ks[k] : cconj(a.ks[L - k]));
         } else {
-            for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], K[pp(k)]);
+            for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k) * des], K[pp(k) * des]);
         }
         if (threadIdx.x == 0) W[0].y = 0;
     } else if (a.ks) {
@@ -1085,10 +1110,9 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
         // would turn both into flat accesses
         for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], a.ks[k]);
     } else {
-        for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], K[pp(k)]);
+        for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k) * des], K[pp(k) * des]);
     }
     __syncthreads();
-    C* other = (W == f0) ? f1 : f0;
     const int ci = fft_lines<R, false>(W, other, a.pl, tw, 1, pp_len(L), 1, true, T);
     const C* Z = ci ? other : W;
     const int nout = int(a.lastsel.count);
@@ -1480,10 +1504,12 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         ctx.err = "a kept kernel spectrum needs LDS-resident rows";
         return NDC_ERR_UNSUPPORTED;
     }
-    // rank 1, line fits three buffers: one fused launch
+    // rank 1, line fits three buffers: one fused launch (four: the kernel and
+    // data lines transformed as one batch)
     if (!ks && N == 1 && !big && (size_t(rowpl.ntw) + size_t(3) * pp_len(L)) * es <= lds) {
         Fused1dArgs<R, E> a;
         std::memset(&a, 0, sizeof(a));
+        a.joint = (size_t(rowpl.ntw) + size_t(4) * pp_len(L)) * es <= lds ? 1 : 0;
         a.pad = pad;
         a.ker = ker;
         a.y = y;
@@ -1493,7 +1519,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         a.lastsel = AxisSel{p.kdil[0] - 1, p.stride[0], p.out[0]};
         a.scale = scale;
         a.pl = rowpl;
-        const size_t smem = (size_t(rowpl.ntw) + size_t(3) * pp_len(L)) * es;
+        const size_t smem = (size_t(rowpl.ntw) + size_t(a.joint ? 4 : 3) * pp_len(L)) * es;
         ctx.kbegin("fft_fused_1d", esz * double(p.x_numel + p.k_numel + p.y_numel), 15.0 * L * std::log2(double(L)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&fused1d_kernel<R, E>), smem)) return NDC_ERR_UNSUPPORTED;
         NDC_LAUNCH(ctx, ctx.stream(), (fused1d_kernel<R, E>), dim3(1), dim3(threads_for(L)), smem, a);
