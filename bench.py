@@ -321,7 +321,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="uneven batch: steps of the unbatched path")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "bench.py measures one GPU only (the library has no multi-GPU path); run with --gpus 1"}))
         sys.exit(2)
     if args.workload == "batch1d":
         run_batch(args)
