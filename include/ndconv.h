@@ -27,6 +27,9 @@
 extern "C" {
 #endif
 
+/* Ranks 1..6: the reference's trait impls need Dim<[Ix; N]>: RemoveAxis
+ * (conv/mod.rs:118-127, conv_fft/mod.rs:198-201), which ndarray provides for
+ * N = 1..6. */
 #define NDC_MAX_RANK 6
 #define NDC_ABI_VERSION 1
 
@@ -168,7 +171,10 @@ const char* ndc_status_string(ndc_status s);
 ndc_status ndc_create(ndc_handle* out);
 ndc_status ndc_destroy(ndc_handle h);
 /* Use an external HIP stream (hipStream_t; NULL is the null stream) for all
- * later calls; ndc_reset_stream returns to the handle's own stream. */
+ * later calls; ndc_reset_stream returns to the handle's own stream.  A switch
+ * orders the new stream after the work already queued on the old one (the
+ * handle's workspace is shared by its calls), so the old stream must still be
+ * valid when the switch is made. */
 ndc_status ndc_set_stream(ndc_handle h, void* hip_stream);
 ndc_status ndc_reset_stream(ndc_handle h);
 ndc_status ndc_synchronize(ndc_handle h);
@@ -186,6 +192,17 @@ ndc_status ndc_plan(const ndc_conv_desc* desc, int32_t op, ndc_plan_info* info);
  * This is the sequential per-axis fill of padding/mod.rs:119-153 and
  * padding/half_dim.rs:30-343 restated as an index map. */
 ndc_status ndc_border_map(const ndc_conv_desc* desc, int32_t op, int32_t axis, int32_t* codes, int64_t len);
+
+/* The kernel launches ndc_conv_fft makes for `desc`, planned on the host
+ * without a device for one with `lds_bytes` of dynamic LDS per workgroup and
+ * `cus` compute units: launch i requests smem[i] bytes of dynamic LDS with
+ * grid[i] workgroups of block[i] threads.  *count receives the number of
+ * launches; at most `cap` are written.  Returns ndc_conv_fft's status for a
+ * plan it cannot launch (NDC_ERR_UNSUPPORTED when a launch would need more
+ * LDS than lds_bytes).  (No reference counterpart: launch-plan introspection
+ * for tests and bindings.) */
+ndc_status ndc_conv_fft_launches(const ndc_conv_desc* desc, int64_t lds_bytes, int32_t cus, int64_t* smem,
+                                 int64_t* grid, int64_t* block, int64_t cap, int64_t* count);
 
 /* ConvExt::conv (conv/mod.rs:110-115, impl :128-200). */
 ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
@@ -262,14 +279,18 @@ typedef enum ndc_batch_algo {
     NDC_BATCH_DIRECT = 2,   /* direct FP32 for every pair */
     NDC_BATCH_PLAIN = 3,    /* one single-problem conv_fft launch sequence per pair (the
                              * unbatched path, for comparison); run_graph runs it eagerly */
-    NDC_BATCH_MFMA = 4,     /* direct on the matrix cores in split BF16x3 (FP32 accumulation)
-                             * for every pair with stride 1 and a dilated kernel of at most
-                             * 1792 taps; the others as NDC_BATCH_AUTO */
-    NDC_BATCH_AUTO_BF16X3 = 5 /* the autotuner's choice among FP32 direct, BF16x3 direct and
-                               * FP32 FFT.  BF16x3 products carry about 16 significant bits:
-                               * on sums of many unrelated terms the normwise error stays near
-                               * 5e-6, but an output dominated by a few input samples can reach
-                               * 3 * 2^-16 relative */
+    NDC_BATCH_MFMA = 4,     /* direct on the matrix cores for every pair with stride 1 and a dilated
+                             * kernel of at most 1024 taps, FP32 accumulation: split BF16x3 when the
+                             * kernel has at least 32 taps, split BF16x6 below; the others as
+                             * NDC_BATCH_AUTO */
+    NDC_BATCH_AUTO_BF16X3 = 5, /* the autotuner's choice among FP32 direct, FP32 FFT and the matrix-core
+                               * forms.  BF16x3 (hi/lo split, three products) carries about 16
+                               * significant bits per product, so it is admitted only for kernels of
+                               * at least 32 taps, where the errors of the many products per output
+                               * average out; shorter kernels may take BF16x6 (three-term split, six
+                               * products, within 5 * 2^-24 per product) */
+    NDC_BATCH_BF16X6 = 6    /* split BF16x6 on the matrix cores for every pair NDC_BATCH_MFMA takes;
+                             * the others as NDC_BATCH_AUTO */
 } ndc_batch_algo;
 
 typedef struct ndc_batch_desc {

@@ -56,6 +56,7 @@ struct CostModel {
     double fft_item[kBatchClasses];
     double dir_a, dir_b;     // FP32 direct item: dir_a + dir_b * outputs * taps
     double mfma_a, mfma_b;   // BF16x3 matrix-core item: mfma_a + mfma_b * K-blocks
+    double mfma6_a, mfma6_b; // BF16x6 matrix-core item, likewise
     bool measured = false;
 };
 
@@ -67,6 +68,8 @@ CostModel default_costs() {
     c.dir_b = 6e-11;
     c.mfma_a = 4e-6;
     c.mfma_b = 4e-7;
+    c.mfma6_a = 4e-6;
+    c.mfma6_b = 8e-7;
     return c;
 }
 
@@ -102,19 +105,34 @@ double fft_cost(const CostModel& cm, const PairShape& ps, int s, int cls) {
 
 constexpr int kMethodDirect = -1;   // FP32 direct items
 constexpr int kMethodMfma = -2;     // BF16x3 matrix-core items
+constexpr int kMethodMfma6 = -3;    // BF16x6 matrix-core items
 
 int mfma_nkb(const PairShape& ps) { return int((ps.kdil + 31 + 15) / 16); }
 
 bool mfma_eligible(const PairShape& ps, int s) { return s == 1 && ps.kdil <= kBatchMfmaMaxKdil; }
 
-double mfma_cost(const CostModel& cm, const PairShape& ps, int s) {
-    if (!mfma_eligible(ps, s)) return INFINITY;
-    const int64_t nt = (ps.out + kBatchMfmaTile - 1) / kBatchMfmaTile;
-    return double(nt) * (cm.mfma_a + cm.mfma_b * double(mfma_nkb(ps)));
+// BF16x3 meets the batch's accuracy only where every output sums enough
+// products of distinct samples for their rounding errors to average (see
+// kBf16x3MinTaps): the kernel needs that many taps, and no window may read one
+// sample many times over, which a signal shorter than the dilated kernel (a
+// circular pad wraps it) or a border repeating one value (Replicate, a
+// nonzero constant) would do.  The three-term split has no such limit.
+bool bf16x3_admitted(const PairShape& ps, bool repeating_border) {
+    return ps.k >= kBf16x3MinTaps && ps.n >= ps.kdil && !repeating_border;
 }
 
-// Method per pair: kMethodDirect, kMethodMfma, or an FFT class.
-int choose(const CostModel& cm, const PairShape& ps, int s, int algo) {
+double mfma_item_cost(const CostModel& cm, const PairShape& ps, bool six) {
+    return six ? cm.mfma6_a + cm.mfma6_b * double(mfma_nkb(ps)) : cm.mfma_a + cm.mfma_b * double(mfma_nkb(ps));
+}
+
+double mfma_cost(const CostModel& cm, const PairShape& ps, int s, bool six) {
+    if (!mfma_eligible(ps, s)) return INFINITY;
+    const int64_t nt = (ps.out + kBatchMfmaTile - 1) / kBatchMfmaTile;
+    return double(nt) * mfma_item_cost(cm, ps, six);
+}
+
+// Method per pair: kMethodDirect, kMethodMfma, kMethodMfma6, or an FFT class.
+int choose(const CostModel& cm, const PairShape& ps, int s, int algo, bool repeating_border) {
     int best_cls = kMethodDirect;
     double best = INFINITY;
     for (int c = 0; c < kBatchClasses; ++c) {
@@ -126,11 +144,15 @@ int choose(const CostModel& cm, const PairShape& ps, int s, int algo) {
     }
     if (algo == NDC_BATCH_DIRECT) return kMethodDirect;
     if (algo == NDC_BATCH_FFT) return best_cls;   // direct when no class holds the dilated kernel
-    if (algo == NDC_BATCH_MFMA && mfma_eligible(ps, s)) return kMethodMfma;
-    int pick = direct_cost(cm, ps, s) <= best ? kMethodDirect : best_cls;
+    if (algo == NDC_BATCH_MFMA && mfma_eligible(ps, s))
+        return bf16x3_admitted(ps, repeating_border) ? kMethodMfma : kMethodMfma6;
+    if (algo == NDC_BATCH_BF16X6 && mfma_eligible(ps, s)) return kMethodMfma6;
+    const double vd = direct_cost(cm, ps, s);
+    int pick = vd <= best ? kMethodDirect : best_cls;
     if (algo == NDC_BATCH_AUTO_BF16X3) {
-        const double vm = mfma_cost(cm, ps, s);
-        if (vm < std::min(best, direct_cost(cm, ps, s))) pick = kMethodMfma;
+        const bool x3 = bf16x3_admitted(ps, repeating_border);
+        const double vm = mfma_cost(cm, ps, s, !x3);
+        if (vm < std::min(best, vd)) pick = x3 ? kMethodMfma : kMethodMfma6;
     }
     return pick;
 }
@@ -180,13 +202,15 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
                 BatchItem it{0, int32_t(i), int32_t(o), cnt};
                 work.emplace_back(cm.dir_a + cm.dir_b * double(cnt) * double(ps.k), it);
             }
-        } else if (method[i] == kMethodMfma) {
+        } else if (method[i] == kMethodMfma || method[i] == kMethodMfma6) {
+            const bool six = method[i] == kMethodMfma6;
             S.main_bytes += 4.0 * double(ps.k);
             S.main_flops += 2.0 * double(ps.out) * double(ps.k);
-            const double c = cm.mfma_a + cm.mfma_b * double(mfma_nkb(ps));
+            const double c = mfma_item_cost(cm, ps, six);
+            const int kind = six ? kBatchKindMfma6 : kBatchKindMfma;
             for (int64_t o = 0; o < ps.out; o += kBatchMfmaTile) {
                 const int cnt = int(std::min<int64_t>(kBatchMfmaTile, ps.out - o));
-                work.emplace_back(c, BatchItem{kBatchKindMfma, int32_t(i), int32_t(o), cnt});
+                work.emplace_back(c, BatchItem{kind, int32_t(i), int32_t(o), cnt});
             }
         } else {
             const int c = method[i];
@@ -354,23 +378,28 @@ bool calibrate(LaunchContext& ctx, int s, int d, CostModel* cm) {
         out.dir_b = std::max(1e-14, (nitems[0] * t[1] - nitems[1] * t[0]) / det);
         out.dir_a = std::max(0.0, (t[0] - work[0] * out.dir_b) / nitems[0]);
     }
-    // matrix-core items (stride 1 only): t = nitems (a + b nkb) at two kernel lengths
+    // matrix-core items (stride 1 only): t = nitems (a + b nkb) at two kernel
+    // lengths, for each split
     if (s == 1) {
         const int64_t kcap = (kBatchMfmaMaxKdil - 1) / d + 1;
-        const int64_t mt[2] = {std::min<int64_t>(64, kcap), std::min<int64_t>(512, kcap)};
-        double tm[2] = {0, 0}, nm[2] = {0, 0}, kbm[2] = {0, 0};
-        for (int j = 0; j < 2; ++j) {
-            auto so = shapes(mt[j]);
-            std::vector<int> method(P, kMethodMfma);
-            Schedule S = build_schedule(so.first, so.second, method, s, out);
-            if (!time_schedule(ctx, S, x, k, y, s, d, &tm[j]) || S.items.empty()) return false;
-            nm[j] = double(S.items.size());
-            kbm[j] = double(mfma_nkb(so.first[0]));
-        }
-        const double pi0 = tm[0] / nm[0], pi1 = tm[1] / nm[1];   // ms per item
-        if (kbm[1] > kbm[0]) {
-            out.mfma_b = std::max(1e-12, (pi1 - pi0) / (kbm[1] - kbm[0]));
-            out.mfma_a = std::max(0.0, pi0 - out.mfma_b * kbm[0]);
+        const int64_t mt[2] = {std::min<int64_t>(8, kcap), std::min<int64_t>(512, kcap)};
+        for (int six = 0; six < 2; ++six) {
+            double tm[2] = {0, 0}, nm[2] = {0, 0}, kbm[2] = {0, 0};
+            for (int j = 0; j < 2; ++j) {
+                auto so = shapes(mt[j]);
+                std::vector<int> method(P, six ? kMethodMfma6 : kMethodMfma);
+                Schedule S = build_schedule(so.first, so.second, method, s, out);
+                if (!time_schedule(ctx, S, x, k, y, s, d, &tm[j]) || S.items.empty()) return false;
+                nm[j] = double(S.items.size());
+                kbm[j] = double(mfma_nkb(so.first[0]));
+            }
+            const double pi0 = tm[0] / nm[0], pi1 = tm[1] / nm[1];   // ms per item
+            if (kbm[1] > kbm[0]) {
+                const double b = std::max(1e-12, (pi1 - pi0) / (kbm[1] - kbm[0]));
+                const double a = std::max(0.0, pi0 - b * kbm[0]);
+                (six ? out.mfma6_b : out.mfma_b) = b;
+                (six ? out.mfma6_a : out.mfma_a) = a;
+            }
         }
     }
     out.measured = true;
@@ -380,7 +409,7 @@ bool calibrate(LaunchContext& ctx, int s, int d, CostModel* cm) {
 
 // The cost model of this handle for strides/dilation (s, d), measured once.
 CostModel cost_model(LaunchContext& ctx, int s, int d, bool measure) {
-    const std::string key = "batch_costs2:" + std::to_string(s) + ":" + std::to_string(d);
+    const std::string key = "batch_costs3:" + std::to_string(s) + ":" + std::to_string(d);
     std::string& m = ctx.memo(key);
     CostModel cm = default_costs();
     if (m.size() == sizeof(CostModel)) {
@@ -465,7 +494,7 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
     if (d->dtype != NDC_F32) return fail(h, NDC_ERR_UNSUPPORTED, "batches are f32");
     if (d->count < 0 || (d->count > 0 && (!d->x_len || !d->k_len)))
         return fail(h, NDC_ERR_INVALID_ARGUMENT, "bad pair count or length arrays");
-    if (d->algo < NDC_BATCH_AUTO || d->algo > NDC_BATCH_AUTO_BF16X3)
+    if (d->algo < NDC_BATCH_AUTO || d->algo > NDC_BATCH_BF16X6)
         return fail(h, NDC_ERR_INVALID_ARGUMENT, "unknown batch algorithm");
     if (d->flags & ~uint32_t(NDC_FLAG_HOST_MEMORY)) return fail(h, NDC_ERR_INVALID_ARGUMENT, "unknown batch flags");
     LaunchContext* ctx = handle_context(h);
@@ -506,7 +535,12 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
             return fail(h, NDC_ERR_UNSUPPORTED, "stride or dilation too large for the batch kernels");
         b->cm = cost_model(*ctx, b->s, b->d, b->algo == NDC_BATCH_AUTO || b->algo == NDC_BATCH_AUTO_BF16X3);
         b->method.resize(size_t(P));
-        for (int64_t i = 0; i < P; ++i) b->method[size_t(i)] = choose(b->cm, b->shapes[size_t(i)], b->s, b->algo);
+        auto repeats = [](const ndc_border& side) {
+            return side.tag == NDC_BORDER_REPLICATE || (side.tag == NDC_BORDER_CONST && side.value.re != 0.0);
+        };
+        const bool rb = repeats(front) || repeats(back);
+        for (int64_t i = 0; i < P; ++i)
+            b->method[size_t(i)] = choose(b->cm, b->shapes[size_t(i)], b->s, b->algo, rb);
         b->S = build_schedule(b->shapes, b->offs, b->method, b->s, b->cm);
         const float2* tw = batch_twiddles(*ctx);
         if (!tw || !b->D.upload(b->S, tw, resident_blocks(*ctx)))
@@ -612,7 +646,9 @@ std::string batch_report(const Batch& b) {
     append(r, ", \"direct_item_ms\": %.4e", b.cm.dir_a);
     append(r, ", \"direct_mac_ms\": %.4e", b.cm.dir_b);
     append(r, ", \"mfma_item_ms\": %.4e", b.cm.mfma_a);
-    append(r, ", \"mfma_kblock_ms\": %.4e}", b.cm.mfma_b);
+    append(r, ", \"mfma_kblock_ms\": %.4e", b.cm.mfma_b);
+    append(r, ", \"mfma6_item_ms\": %.4e", b.cm.mfma6_a);
+    append(r, ", \"mfma6_kblock_ms\": %.4e}", b.cm.mfma6_b);
     // choice per bucket: kernel-length octave x signal-length octave
     std::map<std::pair<int, int>, std::map<std::string, int64_t>> bk;
     for (size_t i = 0; i < b.shapes.size() && i < b.method.size(); ++i) {
@@ -621,6 +657,7 @@ std::string batch_report(const Batch& b) {
         const int m = b.method[i];
         const std::string name = m == kMethodDirect ? "direct_fp32"
                                  : m == kMethodMfma  ? "direct_bf16x3"
+                                 : m == kMethodMfma6 ? "direct_bf16x6"
                                                      : "fft" + std::to_string(2 * batch_lc(m));
         bk[{kb, nb}][name] += 1;
     }

@@ -35,7 +35,17 @@ constexpr int kBatchWindow = 6144;           // LDS window capacity of a direct 
 constexpr int kBatchTapChunk = 1024;         // taps staged per round of a direct item
 constexpr int kBatchMfmaTile = 4096;         // outputs per matrix-core item (1024 per wave)
 constexpr int kBatchMfmaMaxKdil = 1024;      // longest dilated kernel of a matrix-core item (LDS)
-constexpr int kBatchKindMfma = 1 + kBatchClasses;
+constexpr int kBatchKindMfma = 1 + kBatchClasses;    // BF16x3 matrix-core tile (two-term split)
+constexpr int kBatchKindMfma6 = 2 + kBatchClasses;   // BF16x6 matrix-core tile (three-term split)
+// Fewest kernel taps for which the autotuned and matrix-core batch forms use
+// BF16x3.  Its products carry about 16 significant bits (each within
+// 3 * 2^-16 of the exact product); an output summed from a few products keeps
+// that error, while with 32 or more independent products per output the
+// rounding errors average to about 2^-18 of the output scale (normwise 4.7e-6
+// worst in an exact-accumulation model of Uniform[-1,1) data over 1.2 M
+// outputs).  Shorter kernels use the three-term split, whose products are
+// within 5 * 2^-24.
+constexpr int kBf16x3MinTaps = 32;
 
 constexpr int batch_lc(int cls) { return 256 << cls; }
 constexpr int batch_lines(int cls) { return kBatchPoints / batch_lc(cls); }
@@ -58,7 +68,8 @@ struct BatchPair {
 // One unit of the work queue (16 bytes).
 //   kind 0: direct tile  a = pair, b = first output, c = output count
 //   kind 1 + cls: FFT    a = first line, b = line count (<= batch_lines(cls))
-//   kind kBatchKindMfma: BF16x3 matrix-core tile, a = pair, b = first output, c = count
+//   kind kBatchKindMfma / kBatchKindMfma6: BF16x3 / BF16x6 matrix-core tile,
+//       a = pair, b = first output, c = count
 struct BatchItem {
     int32_t kind, a, b, c;
 };

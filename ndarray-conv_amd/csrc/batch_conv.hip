@@ -33,7 +33,21 @@ constexpr int T = kBatchThreads;
 constexpr int kTwSlots = 2048;                                   // L/4 for the largest class
 constexpr int kDataSlots = kBatchPoints + kBatchPoints / 16;     // padded FFT data
 constexpr int kWinSlots = kBatchWindow + kBatchWindow / 8;       // padded direct window
-constexpr size_t kLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
+constexpr size_t kFftLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
+
+__device__ __forceinline__ int padh(int e) { return e + 8 * (e >> 5); }   // 16 bytes of padding per 64
+
+template <int P>
+struct MfmaLds {
+    static constexpr int nkb = (kBatchMfmaMaxKdil + 31 + 15) / 16;
+    static constexpr int win = kBatchMfmaTile + 16 * nkb + 32;         // window samples
+    static constexpr int win_slots = win + 8 * ((win + 31) / 32);      // padded plane length
+    static constexpr int taps = 16 * nkb + 48;                         // tap plane length (even)
+    static constexpr size_t bytes = size_t(P * win_slots + 2 * P * taps) * 2;
+};
+
+// dynamic LDS of both kernels: the larger of the FFT layout and BF16x6 staging
+constexpr size_t kLdsBytes = ((kFftLdsBytes > MfmaLds<3>::bytes ? kFftLdsBytes : MfmaLds<3>::bytes) + 255) & ~size_t(255);
 static_assert(size_t(kWinSlots + kBatchTapChunk) * sizeof(float) <= size_t(kDataSlots) * sizeof(C),
               "direct-item staging must fit the FFT data region");
 
@@ -420,40 +434,28 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
     }
 }
 
-// BF16x3 matrix-core tile (stride 1): outputs [o0, o0 + cnt), cnt <= 4096,
-// 1024 per wave as one 32x32 tile D[n][m] = out[32 n + m] of the product
+// Matrix-core tile (stride 1): outputs [o0, o0 + cnt), cnt <= 4096, 1024 per
+// wave as one 32x32 tile D[n][m] = out[32 n + m] of the product
 //   D = A B,  A[n][j] = pad[32 n + j],  B[j][m] = kc[j - m]   (Toeplitz),
 // K = j running over kdil + 31 values in blocks of 16.  The window and the
-// taps are split hi/lo once, when staged, into separate bf16 planes, so an A
-// operand is one aligned 16-byte read per plane; the taps are kept in two
-// copies shifted by one element so every B read is 4-byte aligned.  Each
-// block issues hi*hi, hi*lo and lo*hi into three independent accumulators.
-__device__ __forceinline__ int padh(int e) { return e + 8 * (e >> 5); }   // 16 bytes of padding per 64
-
-struct MfmaLds {
-    static constexpr int nkb = (kBatchMfmaMaxKdil + 31 + 15) / 16;
-    static constexpr int win = kBatchMfmaTile + 16 * nkb + 32;         // window samples
-    static constexpr int win_slots = win + 8 * ((win + 31) / 32);      // padded plane length
-    static constexpr int taps = 16 * nkb + 48;                         // tap plane length (even)
-    static constexpr size_t bytes = size_t(2 * win_slots + 4 * taps) * 2;
-};
-
-__device__ __forceinline__ uint16_t bf16_hi(uint32_t packed) { return uint16_t(packed & 0xFFFFu); }
-__device__ __forceinline__ uint16_t bf16_lo(uint32_t packed) { return uint16_t(packed >> 16); }
-
+// taps are split into P bf16 terms once, when staged, each term in a plane of
+// its own, so an A operand is one aligned 16-byte read per plane; the taps are
+// kept in two copies shifted by one element so every B read is 4-byte aligned.
+// P = 2 (BF16x3) issues t0*u0, t0*u1, t1*u0 per block; P = 3 (BF16x6) also
+// t1*u1, t2*u0, t0*u2.  Three accumulators keep the chains independent and
+// the small terms apart from the leading one.
+template <int P>
 __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& it, uint16_t* lds) {
+    using L = MfmaLds<P>;
     const BatchPair p = a.pairs[it.a];
     const int o0 = it.b, cnt = it.c;
     const int d = a.d;
     const int kdil = (p.k - 1) * d + 1;
     const int nkb = (kdil + 31 + 15) / 16;
     if (kdil > kBatchMfmaMaxKdil) return;   // the planner never queues these
-    uint16_t* wh = lds;                                   // window, hi plane
-    uint16_t* wl = wh + MfmaLds::win_slots;               // window, lo plane
-    uint16_t* th0 = wl + MfmaLds::win_slots;              // taps hi, tap q at 32 + q
-    uint16_t* th1 = th0 + MfmaLds::taps;                  // taps hi, shifted: tap q at 31 + q
-    uint16_t* tl0 = th1 + MfmaLds::taps;
-    uint16_t* tl1 = tl0 + MfmaLds::taps;
+    uint16_t* wp = lds;                       // window plane t at wp + t * win_slots
+    uint16_t* tp = lds + P * L::win_slots;    // tap plane t, copy c at tp + (2 t + c) * taps:
+                                              // copy 0 holds tap q at 32 + q, copy 1 at 31 + q
     LineCtx c{};
     c.x_off = p.x_off;
     c.n = p.n;
@@ -462,9 +464,10 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     // window: padded positions o0 + e for every sample an A operand reads
     const int W = kBatchMfmaTile + 16 * nkb + 32;
     auto put = [&](int e, float v) {
-        const uint32_t s2 = split_pack(v);
-        wh[padh(e)] = bf16_hi(s2);
-        wl[padh(e)] = bf16_lo(s2);
+        uint16_t t[P];
+        split_terms<P>(v, t);
+#pragma unroll
+        for (int q = 0; q < P; ++q) wp[q * L::win_slots + padh(e)] = t[q];
     };
     if (int64_t(o0) >= c.p0 && int64_t(o0) + W <= int64_t(c.p0) + c.n) {
         const float* src = a.x + c.x_off + (int64_t(o0) - c.p0);   // all interior: no border logic
@@ -472,19 +475,17 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     } else {
         staged<24>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
     }
-    // taps: plane0[e] = kc_dil[e - 32], plane1[e] = kc_dil[e - 31], zero outside [0, kdil)
+    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero outside [0, kdil)
     const int ntp = 16 * nkb + 48;
     auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
-    // one load per tap: plane0[e] and plane1[e - 1] hold the same tap
+    // one load per tap: copy 0 [e] and copy 1 [e - 1] hold the same tap
     staged<5>(ntp + 1, [&](int e) { return tapv(e - 32); }, [&](int e, float v) {
-        const uint32_t s2 = split_pack(v);
-        if (e < ntp) {
-            th0[e] = bf16_hi(s2);
-            tl0[e] = bf16_lo(s2);
-        }
-        if (e >= 1) {
-            th1[e - 1] = bf16_hi(s2);
-            tl1[e - 1] = bf16_lo(s2);
+        uint16_t t[P];
+        split_terms<P>(v, t);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            if (e < ntp) tp[(2 * q) * L::taps + e] = t[q];
+            if (e >= 1) tp[(2 * q + 1) * L::taps + e - 1] = t[q];
         }
     });
     __syncthreads();
@@ -495,52 +496,63 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     // operands of K-block kb: A lane holds row m (a 32-output segment), B lane
     // column m (an output offset); both K entries 16 kb + 8 half + i
     const int par = (32 + 8 * half - m) & 1;   // parity of this lane's first tap index
-    const uint16_t* tbh = par ? th1 : th0;
-    const uint16_t* tbl = par ? tl1 : tl0;
-    auto load = [&](int kb, u32x4& ah, u32x4& al, u32x4& bh, u32x4& bl) {
+    auto load = [&](int kb, u32x4* av, u32x4* bv) {
         const int ea = padh(wb + 32 * m + 16 * kb + 8 * half);
-        ah = *reinterpret_cast<const u32x4*>(wh + ea);
-        al = *reinterpret_cast<const u32x4*>(wl + ea);
         // taps 16 kb + 8 half - m + i, i < 8, from the copy where they start even
         const int q = 32 + 16 * kb + 8 * half - m - par;
-        const uint32_t* ph = reinterpret_cast<const uint32_t*>(tbh + q);
-        const uint32_t* pl = reinterpret_cast<const uint32_t*>(tbl + q);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            bh[i] = ph[i];
-            bl[i] = pl[i];
+        for (int s = 0; s < P; ++s) {
+            av[s] = *reinterpret_cast<const u32x4*>(wp + s * L::win_slots + ea);
+            const uint32_t* pb = reinterpret_cast<const uint32_t*>(tp + (2 * s + par) * L::taps + q);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[s][i] = pb[i];
         }
     };
-    floatx16 acc_hh, acc_hl, acc_lh;
+    auto mfma = [](const u32x4& x, const u32x4& y, floatx16 acc) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, x), __builtin_bit_cast(bf16x8, y),
+                                                       acc, 0, 0, 0);
+    };
+    floatx16 acc0, acc1, acc2;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc_hh[i] = acc_hl[i] = acc_lh[i] = 0.f;
-    u32x4 ah, al, bh, bl;
-    load(0, ah, al, bh, bl);
+    for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = acc2[i] = 0.f;
+    u32x4 av[P], bv[P];
+    load(0, av, bv);
     for (int kb = 0; kb < nkb; ++kb) {
-        u32x4 nah = ah, nal = al, nbh = bh, nbl = bl;
-        if (kb + 1 < nkb) load(kb + 1, nah, nal, nbh, nbl);   // next block's operands in flight
-        acc_hh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah), __builtin_bit_cast(bf16x8, bh),
-                                                        acc_hh, 0, 0, 0);
-        acc_hl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah), __builtin_bit_cast(bf16x8, bl),
-                                                        acc_hl, 0, 0, 0);
-        acc_lh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al), __builtin_bit_cast(bf16x8, bh),
-                                                        acc_lh, 0, 0, 0);
-        ah = nah;
-        al = nal;
-        bh = nbh;
-        bl = nbl;
+        u32x4 na[P], nb[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            na[s] = av[s];
+            nb[s] = bv[s];
+        }
+        if (kb + 1 < nkb) load(kb + 1, na, nb);   // next block's operands in flight
+        acc0 = mfma(av[0], bv[0], acc0);
+        acc1 = mfma(av[0], bv[1], acc1);
+        acc2 = mfma(av[1], bv[0], acc2);
+        if constexpr (P == 3) {
+            acc1 = mfma(av[1], bv[1], acc1);
+            acc2 = mfma(av[2], bv[0], acc2);
+            acc1 = mfma(av[0], bv[2], acc1);
+        }
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            av[s] = na[s];
+            bv[s] = nb[s];
+        }
     }
     // D lane holds D[8 (i/4) + 4 half + i%4][m]
     float* y = a.y + p.y_off + o0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int o = wb + 32 * (8 * (i / 4) + 4 * half + (i % 4)) + m;
-        const float v = acc_hh[i] + (acc_hl[i] + acc_lh[i]);
+        const float v = acc0[i] + (acc1[i] + acc2[i]);
         if (o < cnt) y[o] = v;
     }
 }
 
-static_assert(MfmaLds::bytes <= size_t(kDataSlots) * sizeof(C), "matrix-core staging must fit the FFT data region");
+// BF16x3 staging fits the FFT data region (the twiddle table stays); BF16x6
+// staging takes the whole dynamic allocation, twiddles included.
+static_assert(MfmaLds<2>::bytes <= size_t(kDataSlots) * sizeof(C), "BF16x3 staging must fit the FFT data region");
+static_assert(MfmaLds<3>::bytes <= kLdsBytes, "BF16x6 staging must fit the dynamic LDS");
 
 __device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
     for (int i = tid(); i < n; i += T) dst[i] = src[i];
@@ -592,7 +604,10 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         if (it.kind == 0) {
             direct_item(a, it, win, taps);
         } else if (it.kind == kBatchKindMfma) {
-            mfma_item(a, it, reinterpret_cast<uint16_t*>(buf));
+            mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf));
+        } else if (it.kind == kBatchKindMfma6) {
+            mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem));
+            cur = -1;   // the staging overwrote the twiddle table
         } else {
             const int cls = it.kind - 1;
             if (cls != cur) {

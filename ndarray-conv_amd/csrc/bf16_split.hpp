@@ -1,5 +1,7 @@
-// BF16 operands for the matrix cores: round-to-nearest-even conversion and
-// the hi/lo split of the "BF16x3" products (a*b ~ ah*bh + ah*bl + al*bh).
+// BF16 operands for the matrix cores: round-to-nearest-even conversion, the
+// hi/lo split of the "BF16x3" products (a*b ~ ah*bh + ah*bl + al*bh) and the
+// three-term split of the "BF16x6" products (a*b ~ a0b0 + a0b1 + a1b0 + a0b2 +
+// a1b1 + a2b0).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +26,21 @@ __device__ __forceinline__ uint32_t split_pack(float f) {
     const float hf = __builtin_bit_cast(float, h << 16);
     const uint32_t l = bf16_bits(f - hf);
     return h | (l << 16);
+}
+
+// f = t[0] + ... + t[P-1] (+ a residual of at most 2^-(8P) |f|): each term is
+// the bf16 rounding of what the previous terms left, and every difference is
+// exact in FP32.  P = 2 is the hi/lo split of BF16x3; P = 3 is the three-term
+// split of BF16x6.
+template <int P>
+__device__ __forceinline__ void split_terms(float f, uint16_t* t) {
+    float r = f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const uint32_t h = bf16_bits(r);
+        t[p] = uint16_t(h);
+        if (p + 1 < P) r -= __builtin_bit_cast(float, h << 16);
+    }
 }
 
 // Eight packed (hi | lo << 16) values to the two 8-element bf16 operands:

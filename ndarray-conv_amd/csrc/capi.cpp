@@ -37,6 +37,26 @@ public:
     std::unordered_map<std::string, Plan> plans;   // keyed by the raw descriptor bytes and op
     std::string last;
     int64_t rp_origin_len = 0;   // real length of the last axis of the last real forward transform
+    hipEvent_t order_ev = nullptr;   // orders a newly selected stream after the previous one
+
+    // Makes `next` the stream of later calls.  The handle's workspace (spectra,
+    // io buffers, batch queues) is reused by every call, so work already queued
+    // on the stream being left must finish before `next` runs anything: an
+    // event recorded on the old stream is waited for by the new one.
+    ndc_status switch_stream(hipStream_t next, bool external) {
+        hipStream_t prev = stream();
+        if (prev != next || use_ext != external) {
+            if (!order_ev && hipEventCreateWithFlags(&order_ev, hipEventDisableTiming) != hipSuccess) {
+                order_ev = nullptr;
+                return NDC_ERR_DEVICE;
+            }
+            if (hipEventRecord(order_ev, prev) != hipSuccess || hipStreamWaitEvent(next, order_ev, 0) != hipSuccess)
+                return NDC_ERR_DEVICE;
+        }
+        ext = external ? next : nullptr;
+        use_ext = external;
+        return NDC_OK;
+    }
 
     // per-kernel profile: HIP events recorded on the handle's stream around
     // every launch while enabled
@@ -57,6 +77,7 @@ public:
         for (auto& t : tables) (void)hipFree(t.second.ptr);
         for (auto& r : recs) { pool.push_back(r.a); pool.push_back(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (order_ev) (void)hipEventDestroy(order_ev);
         if (own) (void)hipStreamDestroy(own);
     }
 
@@ -206,6 +227,42 @@ public:
         for (int i = 0; i <= kMaxRank; ++i) offsets[i] = it->second.second[i];
         return static_cast<const int32_t*>(it->second.first.ptr);
     }
+};
+
+// Plans a call's launches on the host without a device (LaunchContext::dry_run):
+// device memory requests get placeholders, launches are recorded.
+class DryRunContext final : public LaunchContext {
+public:
+    DryRunContext(size_t lds, int cus) : lds_(lds), cus_(cus) {}
+    struct Launch {
+        std::string kernel;
+        size_t smem;
+        unsigned grid, block;
+    };
+    std::vector<Launch> launches;
+
+    hipStream_t stream() override { return nullptr; }
+    void* scratch(int slot, size_t) override { return placeholder(slot); }
+    const void* twiddles(int64_t, bool) override { return placeholder(8); }
+    const void* constant_table(const std::string&, const void*, size_t) override { return placeholder(9); }
+    std::string& memo(const std::string& key) override { return memos_[key]; }
+    const int32_t* border_maps(const Plan&, int64_t offsets[kMaxRank + 1]) override {
+        for (int i = 0; i <= kMaxRank; ++i) offsets[i] = 0;
+        return static_cast<const int32_t*>(placeholder(10));
+    }
+    int cu_count() override { return cus_; }
+    size_t max_lds() override { return lds_; }
+    bool dry_run() override { return true; }
+    void record_launch(const char* kernel, size_t smem, unsigned grid, unsigned block) override {
+        launches.push_back(Launch{kernel, smem, grid, block});
+    }
+
+private:
+    // distinct, aligned, never dereferenced
+    static void* placeholder(int i) { return reinterpret_cast<void*>(uintptr_t(i + 1) << 20); }
+    size_t lds_;
+    int cus_;
+    std::unordered_map<std::string, std::string> memos_;
 };
 
 ndc_status set_err(ndc_handle h, ndc_status s, const std::string& msg) {
@@ -515,17 +572,15 @@ ndc_status ndc_destroy(ndc_handle h) {
 ndc_status ndc_set_stream(ndc_handle h, void* stream) {
     if (!h) return NDC_ERR_INVALID_ARGUMENT;
     Handle* p = reinterpret_cast<Handle*>(h);
-    p->ext = static_cast<hipStream_t>(stream);
-    p->use_ext = true;
-    return NDC_OK;
+    if (hipSetDevice(p->device) != hipSuccess) return NDC_ERR_DEVICE;
+    return p->switch_stream(static_cast<hipStream_t>(stream), true);
 }
 
 ndc_status ndc_reset_stream(ndc_handle h) {
     if (!h) return NDC_ERR_INVALID_ARGUMENT;
     Handle* p = reinterpret_cast<Handle*>(h);
-    p->ext = nullptr;
-    p->use_ext = false;
-    return NDC_OK;
+    if (hipSetDevice(p->device) != hipSuccess) return NDC_ERR_DEVICE;
+    return p->switch_stream(p->own, false);
 }
 
 ndc_status ndc_synchronize(ndc_handle h) {
@@ -568,6 +623,31 @@ ndc_status ndc_border_map(const ndc_conv_desc* desc, int32_t op, int32_t axis, i
     const auto& m = p.map[axis];
     if (len != int64_t(m.size())) return NDC_ERR_INVALID_ARGUMENT;
     std::memcpy(codes, m.data(), m.size() * sizeof(int32_t));
+    return NDC_OK;
+}
+
+ndc_status ndc_conv_fft_launches(const ndc_conv_desc* desc, int64_t lds_bytes, int32_t cus, int64_t* smem,
+                                 int64_t* grid, int64_t* block, int64_t cap, int64_t* count) {
+    if (!desc || lds_bytes <= 0 || cus <= 0 || cap < 0 || !count || (cap > 0 && (!smem || !grid || !block)))
+        return NDC_ERR_INVALID_ARGUMENT;
+    ndc::Plan p;
+    std::string err;
+    ndc_status s = ndc::make_plan(desc, NDC_OP_CONV_FFT, &p, &err, false);
+    if (s != NDC_OK) return s;
+    ndc::DryRunContext ctx(size_t(lds_bytes), cus);
+    *count = 0;
+    if (p.y_numel > 0) {
+        s = ndc::launch_fft(p, reinterpret_cast<const void*>(uintptr_t(1) << 24),
+                            reinterpret_cast<const void*>(uintptr_t(2) << 24), reinterpret_cast<void*>(uintptr_t(3) << 24),
+                            ctx);
+        if (s != NDC_OK) return s;
+    }
+    *count = int64_t(ctx.launches.size());
+    for (int64_t i = 0; i < cap && i < *count; ++i) {
+        smem[i] = int64_t(ctx.launches[size_t(i)].smem);
+        grid[i] = int64_t(ctx.launches[size_t(i)].grid);
+        block[i] = int64_t(ctx.launches[size_t(i)].block);
+    }
     return NDC_OK;
 }
 

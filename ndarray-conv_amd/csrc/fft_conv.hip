@@ -712,22 +712,31 @@ struct AxisSel {
 template <typename R>
 struct ColsArgs {
     Cx<R>* spec[2];
-    int64_t sstr[kMaxRank];          // spectrum strides (complex elements)
-    AxisSel outer[2][kMaxRank];      // selection per outer axis per job (axes != d, < N-1)
-    int32_t outer_axis[kMaxRank];    // which spectrum axis each outer index is
+    int64_t sd;                      // element stride between consecutive points of a line
+    int64_t ostr[kMaxRank];          // element stride of each outer index
+    AxisSel outer[2][kMaxRank];      // selection per outer index per job
     int32_t nouter;
     int64_t nlines_outer[2];         // product of outer counts per job
     int32_t ntiles;                  // bin tiles along the last axis
     int32_t nblocks0;
     int32_t H;                       // bins
     int32_t C;                       // tile width (lines per block)
-    int32_t d;                       // transformed axis
     int32_t valid[2];                // rows of axis d holding data (rest are zero)
     AxisSel outsel;                  // rows of axis d to write back
     int32_t mode;                    // 0 forward (2 jobs), 1 fused forward*kernel->inverse, 2 inverse,
                                      // 3 fused with the kernel spectrum already transformed
     int32_t nblocks;                 // grid size
     int32_t joint;                   // mode 1: kernel and data columns transformed as one batch
+    // Four-step split of a column axis of length F = L1 * L2 (point j at
+    // j1 L2 + j2): the L1-point pass runs along j1 with j2 the outer index
+    // tw_outer.  Rows j1 with j1 L2 + j2 >= valid are zero (split_L2 = L2;
+    // 0 when the line is a whole axis).  tw_dir 1 multiplies its forward
+    // output k1 by W_F^(k1 j2), 2 multiplies its inverse input by the
+    // conjugate; twF holds W_F^i, i < F.
+    const Cx<R>* twF;
+    int32_t split_L2;
+    int32_t tw_outer;
+    int32_t tw_dir;
     LinePlan<R> pl;
 };
 
@@ -760,20 +769,30 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     const int tile = int(blk % a.ntiles);
     int64_t rem = blk / a.ntiles;
     int64_t base = 0;
+    int j2 = 0;   // the split's outer coordinate (split passes only)
     for (int i = a.nouter - 1; i >= 0; --i) {
         const AxisSel& s = a.outer[job][i];
         const int64_t q = rem % s.count;
         rem /= s.count;
-        base += (s.start + q * s.step) * a.sstr[a.outer_axis[i]];
+        const int64_t pos = s.start + q * s.step;
+        if (i == a.tw_outer) j2 = int(pos);
+        base += pos * a.ostr[i];
     }
     const int c0 = tile * Ct;
     const int nc = min(Ct, a.H - c0);
-    const int64_t sd = a.sstr[a.d];
+    const int64_t sd = a.sd;
     const float rC = 1.0f / float(Ct);
+    // rows of this line holding data: j1 L2 + j2 < valid in a split pass
+    auto valid_rows = [&](int valid) {
+        if (a.split_L2 <= 0) return valid;
+        return valid > j2 ? (valid - j2 + a.split_L2 - 1) / a.split_L2 : 0;
+    };
+    const bool pre_tw = a.tw_dir == 2;
 
     // tile column c of row t goes to dstbuf[pp(t) * es + c] (es = Ct, or 2 Ct
     // in the fused mode's joint kernel + data buffer)
     auto load = [&](const C* src, C* dstbuf, int valid, int es) {
+        valid = valid_rows(valid);
         const int total = L * Ct;
         for (int b = threadIdx.x; b < total; b += kU * T) {
             C v[kU];
@@ -784,7 +803,10 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
                 v[u] = cmake<C>(R(0), R(0));
                 const int t = fdiv(it, Ct, rC), c = it - t * Ct;
                 at[u] = pp(t) * es + c;
-                if (it < total && t < valid && c < nc) v[u] = src[base + int64_t(t) * sd + c0 + c];
+                if (it < total && t < valid && c < nc) {
+                    v[u] = src[base + int64_t(t) * sd + c0 + c];
+                    if (pre_tw) v[u] = cmul(v[u], cconj(a.twF[t * j2]));
+                }
             }
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
@@ -856,11 +878,14 @@ __global__ void __launch_bounds__(kMaxThreads) cols_kernel(ColsArgs<R> a) {
     // store selected rows
     C* dst = a.spec[job];
     const int nrow = int(a.outsel.count);
+    const bool post_tw = a.tw_dir == 1;
     for (int it = threadIdx.x; it < nrow * Ct; it += T) {
         const int i = fdiv(it, Ct, rC), c = it - i * Ct;
         if (c >= nc) continue;
         const int t = int(a.outsel.start + int64_t(i) * a.outsel.step);
-        dst[base + int64_t(t) * sd + c0 + c] = res[pp(t) * res_es + c];
+        C v = res[pp(t) * res_es + c];
+        if (post_tw) v = cmul(v, a.twF[t * j2]);
+        dst[base + int64_t(t) * sd + c0 + c] = v;
     }
 }
 
@@ -1414,6 +1439,7 @@ template <> float2 host_value<float2>(const ndc_value& v) { return cmake<float2>
 template <> double2 host_value<double2>(const ndc_value& v) { return cmake<double2>(v.re, v.im); }
 
 ndc_status check_launch(LaunchContext& ctx, const char* what) {
+    if (ctx.dry_run()) return NDC_OK;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx.err = std::string(what) + ": " + hipGetErrorString(e);
@@ -1721,7 +1747,7 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     }
 
     // ---- pass 1: last axis forward
-    const bool data_job = !(ks && ks->bake), kernel_job = !(ks && !ks->bake);
+    bool data_job = !(ks && ks->bake), kernel_job = !(ks && !ks->bake);
     if (big) {
         ndc_status s = big_col(false);
         if (s == NDC_OK) s = big_row(0);
@@ -1731,42 +1757,42 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         if (s != NDC_OK) return s;
     }
 
-    auto col_pass = [&](int d, int cmode) -> ndc_status {
+    // One column-pass launch: lines of Ld points at element stride sdl, one
+    // per outer position of `outer` and bin tile (see ColsArgs).
+    struct OuterAx {
+        int64_t stride;
+        AxisSel sel[2];
+    };
+    const Cx<R>* twF = nullptr;   // W_F^i of the axis being split, when one is
+    auto cols_launch = [&](int Ld, int64_t sdl, const std::vector<OuterAx>& outer, int cmode, int valid0,
+                           int valid1, AxisSel outsel, int split_L2, int tw_outer, int tw_dir) -> ndc_status {
         ColsArgs<R> a;
         std::memset(&a, 0, sizeof(a));
         a.spec[0] = Sx;
         a.spec[1] = Sk;
-        for (int e = 0; e < N; ++e) a.sstr[e] = sstr[e];
-        a.d = d;
+        a.sd = sdl;
         a.H = H;
         a.mode = cmode;
-        const int Ld = int(p.fft[d]);
         if (!make_lineplan<R>(&a.pl, Ld, 0, ctx)) {
             ctx.err = "twiddle table allocation failed";
             return NDC_ERR_OUT_OF_MEMORY;
         }
-        int no = 0;
-        for (int e = 0; e < last; ++e) {
-            if (e == d) continue;
-            a.outer_axis[no] = e;
-            for (int j = 0; j < 2; ++j) {
-                AxisSel s{0, 1, p.fft[e]};
-                if (cmode == 0 && e < d) s = AxisSel{0, 1, j == 0 ? p.pds[e] : p.kdil[e]};
-                if (cmode == 2 && e < d) s = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
-                a.outer[j][no] = s;
-            }
-            ++no;
-        }
-        a.nouter = no;
+        a.nouter = int(outer.size());
         a.nlines_outer[0] = a.nlines_outer[1] = 1;
-        for (int i = 0; i < no; ++i) {
-            a.nlines_outer[0] *= a.outer[0][i].count;
-            a.nlines_outer[1] *= a.outer[1][i].count;
+        for (int i = 0; i < a.nouter; ++i) {
+            a.ostr[i] = outer[size_t(i)].stride;
+            for (int j = 0; j < 2; ++j) {
+                a.outer[j][i] = outer[size_t(i)].sel[j];
+                a.nlines_outer[j] *= outer[size_t(i)].sel[j].count;
+            }
         }
-        a.valid[0] = int(cmode == 2 ? Ld : p.pds[d]);
-        a.valid[1] = int(p.kdil[d]);
-        if (cmode == 0) a.outsel = AxisSel{0, 1, Ld};
-        else a.outsel = AxisSel{p.kdil[d] - 1, p.stride[d], p.out[d]};
+        a.valid[0] = valid0;
+        a.valid[1] = valid1;
+        a.outsel = outsel;
+        a.split_L2 = split_L2;
+        a.tw_outer = tw_outer;
+        a.tw_dir = tw_dir;
+        a.twF = twF;
         if (cmode == 0 && !data_job) a.nlines_outer[0] = 0;
         if (cmode == 0 && !kernel_job) a.nlines_outer[1] = 0;
         // fused: two joint kernel + data buffers (4 tiles) when they fit the
@@ -1792,21 +1818,114 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
         const int64_t nb1 = cmode == 0 ? a.nlines_outer[1] * a.ntiles : 0;
         a.nblocks0 = int32_t(nb0);
         a.nblocks = int32_t(nb0 + nb1);
+        if (nb0 + nb1 == 0) return NDC_OK;
         // compulsory traffic: the rows read (valid / all) and the rows written
         double lines0 = double(a.nlines_outer[0]) * H, lines1 = double(a.nlines_outer[1]) * H;
+        const double v0 = split_L2 > 0 ? double(valid0) / split_L2 : double(valid0);
+        const double v1 = split_L2 > 0 ? double(valid1) / split_L2 : double(valid1);
         double bytes = 0, nfft = 0;
-        if (cmode == 0) { bytes = (lines0 * (a.valid[0] + Ld) + lines1 * (a.valid[1] + Ld)); nfft = lines0 + lines1; }
-        else if (cmode == 1) { bytes = lines0 * (a.valid[0] + a.valid[1] + double(a.outsel.count)); nfft = 3 * lines0; }
-        else if (cmode == 3) { bytes = lines0 * (a.valid[0] + Ld + double(a.outsel.count)); nfft = 2 * lines0; }
+        if (cmode == 0) { bytes = (lines0 * (std::min<double>(v0, Ld) + Ld) + lines1 * (std::min<double>(v1, Ld) + Ld)); nfft = lines0 + lines1; }
+        else if (cmode == 1) { bytes = lines0 * (v0 + v1 + double(a.outsel.count)); nfft = 3 * lines0; }
+        else if (cmode == 3) { bytes = lines0 * (v0 + Ld + double(a.outsel.count)); nfft = 2 * lines0; }
         else { bytes = lines0 * (Ld + double(a.outsel.count)); nfft = lines0; }
         bytes *= double(es);
         static const char* names[4] = {"fft_cols_fwd", "fft_cols_fused", "fft_cols_inv", "fft_cols_fused_kept"};
-        ctx.kbegin(names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
+        static const char* split_names[4] = {"fft_cols_split_fwd", "fft_cols_split_fused", "fft_cols_split_inv",
+                                             "fft_cols_split_fused_kept"};
+        const bool split = tw_outer >= 0;
+        ctx.kbegin(split ? split_names[cmode] : names[cmode], bytes, 5.0 * nfft * Ld * std::log2(double(Ld)));
         if (!lds_fits(ctx, reinterpret_cast<const void*>(&cols_kernel<R>), smem)) return NDC_ERR_UNSUPPORTED;
         NDC_LAUNCH(ctx, ctx.stream(), (cols_kernel<R>),
             dim3(unsigned(nb0 + nb1)), dim3(threads_for(int64_t(Ld) * Ct / 2)), smem, a);
         ctx.kend();
         return check_launch(ctx, "column pass");
+    };
+
+    // A column line of F points fits one workgroup when the fused pass's
+    // three tile buffers of one column fit the LDS budget.
+    auto col_fits = [&](int64_t F) {
+        return F <= (int64_t(1) << 24) &&
+               (size_t(ntw_of(int(F), 0, kDbl)) + size_t(3) * pp_len(int(F))) * es <= std::min<size_t>(lds, 96 * 1024);
+    };
+    // F = L1 * L2 with both factors fitting, as close to sqrt(F) as possible
+    // (0 when no such split exists)
+    auto col_split = [&](int64_t F) -> int64_t {
+        int64_t best = 0;
+        for (int64_t l2 = 2; l2 * l2 <= F; ++l2) {
+            if (F % l2) continue;
+            const int64_t l1 = F / l2;
+            if (col_fits(l1) && col_fits(l2)) best = l2;
+        }
+        return best;
+    };
+
+    // Axis d of the spectrum, one of the column modes 0..3 (see ColsArgs):
+    // one launch when its lines fit a workgroup, otherwise the four-step
+    // split F = L1 * L2 (point j1 L2 + j2, spectral index k1 + L1 k2 kept at
+    // k1 L2 + k2, which the elementwise product does not care about): forward
+    // L1-point passes along j1 with a W_F^(k1 j2) post-multiply, then
+    // L2-point passes along j2; the inverse mirrors it and restores natural
+    // order, every row written (later passes read the rows they select).
+    auto col_pass = [&](int d, int cmode) -> ndc_status {
+        const int64_t F = p.fft[d];
+        std::vector<OuterAx> outer;
+        for (int e = 0; e < last; ++e) {
+            if (e == d) continue;
+            OuterAx o;
+            o.stride = sstr[e];
+            for (int j = 0; j < 2; ++j) {
+                AxisSel s{0, 1, p.fft[e]};
+                if (cmode == 0 && e < d) s = AxisSel{0, 1, j == 0 ? p.pds[e] : p.kdil[e]};
+                if (cmode == 2 && e < d) s = AxisSel{p.kdil[e] - 1, p.stride[e], p.out[e]};
+                o.sel[j] = s;
+            }
+            outer.push_back(o);
+        }
+        const int valid0 = int(cmode == 2 ? F : p.pds[d]);
+        const int valid1 = int(p.kdil[d]);
+        const AxisSel crop{p.kdil[d] - 1, p.stride[d], p.out[d]};
+        if (col_fits(F))
+            return cols_launch(int(F), sstr[d], outer, cmode, valid0, valid1, cmode == 0 ? AxisSel{0, 1, F} : crop, 0,
+                               -1, 0);
+        const int64_t L2 = col_split(F);
+        if (!L2) {
+            ctx.err = "FFT length of a column axis cannot be split into LDS-resident factors";
+            return NDC_ERR_UNSUPPORTED;
+        }
+        const int64_t L1 = F / L2;
+        if (!(twF = static_cast<const Cx<R>*>(ctx.twiddles(F, kDbl)))) {
+            ctx.err = "twiddle table allocation failed";
+            return NDC_ERR_OUT_OF_MEMORY;
+        }
+        const int jax = int(outer.size());   // the extra outer index: j2 (A passes) or k1 (B passes)
+        std::vector<OuterAx> outA = outer, outB = outer;
+        outA.push_back(OuterAx{sstr[d], {AxisSel{0, 1, L2}, AxisSel{0, 1, L2}}});
+        outB.push_back(OuterAx{L2 * sstr[d], {AxisSel{0, 1, L1}, AxisSel{0, 1, L1}}});
+        const AxisSel allA{0, 1, L1}, allB{0, 1, L2};
+        ndc_status s = NDC_OK;
+        if (cmode == 2) {
+            s = cols_launch(int(L2), sstr[d], outB, 2, int(L2), int(L2), allB, 0, -1, 0);
+            if (s == NDC_OK) s = cols_launch(int(L1), L2 * sstr[d], outA, 2, int(L1), int(L1), allA, 0, jax, 2);
+            return s;
+        }
+        // forward along j1 (the data and kernel rows holding values), twiddled
+        const bool kjob = kernel_job && cmode != 3;
+        const bool djob = data_job;
+        {
+            const bool sd0 = data_job, sk0 = kernel_job;
+            data_job = djob;
+            kernel_job = kjob;
+            s = cols_launch(int(L1), L2 * sstr[d], outA, 0, valid0, valid1, allA, int(L2), jax, 1);
+            data_job = sd0;
+            kernel_job = sk0;
+        }
+        if (s != NDC_OK) return s;
+        if (cmode == 0) return cols_launch(int(L2), sstr[d], outB, 0, int(L2), int(L2), allB, 0, -1, 0);
+        // along j2: forward, product, inverse (cmode 1 or 3), every row kept
+        s = cols_launch(int(L2), sstr[d], outB, cmode, int(L2), int(L2), allB, 0, -1, 0);
+        if (s != NDC_OK) return s;
+        // inverse along j1 with the conjugate twiddle, natural order again
+        return cols_launch(int(L1), L2 * sstr[d], outA, 2, int(L1), int(L1), allA, 0, jax, 2);
     };
 
     // ---- pass 2: axes N-2 .. 1 forward
@@ -2206,6 +2325,29 @@ size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds) {
     int64_t n = Hp;
     for (int e = 0; e < last; ++e) n *= p.fft[e];
     return size_t(n) * es;
+}
+
+size_t fft_workspace_bytes(const Plan& p, size_t lds) {
+    const size_t es = (p.dtype == NDC_F64 || p.dtype == NDC_C128) ? 16 : 8;
+    const int last = p.N - 1;
+    const int64_t Fl = p.fft[last];
+    // the two spectra: half-spectrum rows padded to 8 bins, or full rows of
+    // the four-step form (all Fl bins kept)
+    size_t spec = fft_kernel_spectrum_bytes(p, lds);
+    const bool big = spec == 0;
+    if (big) {
+        int64_t n = Fl;
+        for (int e = 0; e < last; ++e) n *= p.fft[e];
+        spec = size_t(n) * es;
+    }
+    // stage and post-processing tables: at most 2 L + 257 + 64 entries per
+    // axis plan (sum over stages of Ns (P - 1) is L - 1, generic radices add
+    // P <= 257 each, post-processing L + 1); the four-step form adds its two
+    // factor plans and an Fl-entry table
+    size_t tables = 0;
+    for (int e = 0; e <= last; ++e) tables += (2 * size_t(p.fft[e]) + 600) * es + 256;
+    if (big) tables += (3 * size_t(Fl) + 1200) * es + 768;
+    return 2 * spec + tables + 512;
 }
 
 ndc_status launch_fft_processor(int dtype, bool forward, int N, const int64_t* shape, int64_t origin_len,

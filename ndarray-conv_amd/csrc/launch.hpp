@@ -49,6 +49,13 @@ public:
     virtual void kbegin(const char* name, double bytes, double flops) { (void)name; (void)bytes; (void)flops; }
     virtual bool kevents(hipEvent_t* start, hipEvent_t* stop) { (void)start; (void)stop; return false; }
     virtual void kend() {}
+    // A dry run plans every launch of a call without a device: NDC_LAUNCH
+    // hands the launch to record_launch instead of the runtime, and device
+    // memory requests return placeholders that are never dereferenced.
+    virtual bool dry_run() { return false; }
+    virtual void record_launch(const char* kernel, size_t smem, unsigned grid, unsigned block) {
+        (void)kernel; (void)smem; (void)grid; (void)block;
+    }
     std::string err;
 };
 
@@ -59,7 +66,9 @@ public:
 #define NDC_LAUNCH(ctx, stream, kernel, grid, block, smem, ...)                                           \
     do {                                                                                                  \
         hipEvent_t ndc_e0_ = nullptr, ndc_e1_ = nullptr;                                                  \
-        if ((ctx).kevents(&ndc_e0_, &ndc_e1_))                                                            \
+        if ((ctx).dry_run())                                                                              \
+            (ctx).record_launch(#kernel, size_t(smem), dim3(grid).x, dim3(block).x);                      \
+        else if ((ctx).kevents(&ndc_e0_, &ndc_e1_))                                                       \
             hipExtLaunchKernelGGL(kernel, grid, block, uint32_t(smem), stream, ndc_e0_, ndc_e1_, 0,       \
                                   __VA_ARGS__);                                                           \
         else                                                                                              \
@@ -70,6 +79,11 @@ public:
 // a launch that would not fit is refused on the host instead of reaching the
 // device.  The static size is looked up once per kernel.
 inline bool lds_fits(LaunchContext& ctx, const void* kernel, size_t dynamic_bytes) {
+    if (ctx.dry_run()) {
+        if (dynamic_bytes <= ctx.max_lds()) return true;
+        ctx.err = "kernel LDS request exceeds the workgroup limit";
+        return false;
+    }
     thread_local std::unordered_map<const void*, size_t> cache;
     auto it = cache.find(kernel);
     if (it == cache.end()) {
@@ -107,6 +121,11 @@ ndc_status launch_fft(const Plan& p, const void* x, const void* k, void* y, Laun
 // Bytes of a kept kernel spectrum for the plan, or 0 when its rows are too
 // long for one workgroup (those plans recompute the kernel spectrum per call).
 size_t fft_kernel_spectrum_bytes(const Plan& p, size_t lds);
+
+// Upper bound of the device memory launch_fft (without a kept spectrum)
+// requests through LaunchContext::scratch, twiddles and constant_table for
+// the plan, alignment included.
+size_t fft_workspace_bytes(const Plan& p, size_t lds);
 
 // The processor's plain N-d transforms (processor/real.rs, complex.rs) on
 // contiguous device arrays.  dtype F32/F64: real input (forward) or output

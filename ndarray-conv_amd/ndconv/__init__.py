@@ -33,7 +33,7 @@ __all__ = ["ConvMode", "PaddingMode", "BorderType", "KernelWithDilation", "with_
            "no_reverse", "conv", "conv_fft", "conv_fft_with_processor", "conv_fft_par", "get_fft_processor",
            "FftProcessor", "plan", "border_map", "Error", "DataShape", "KernelShape", "MismatchShape",
            "InvalidArgument", "Unsupported", "DeviceError", "BatchPlan", "conv_fft_batch", "batch_border_map",
-           "Baked", "conv_fft_bake", "conv_fft_with_baked"]
+           "Baked", "conv_fft_bake", "conv_fft_with_baked", "fft_launch_plan"]
 
 
 # ---------------------------------------------------------------------------
@@ -485,6 +485,22 @@ def border_map(data_shape, kernel_shape, conv_mode, padding_mode, axis, dilation
     return out
 
 
+def fft_launch_plan(data_shape, kernel_shape, conv_mode, padding_mode=PaddingMode.Zeros, dilation=1,
+                    reverse_flag=True, dtype=np.float32, lds_bytes=160 * 1024 - 1024, cus=256):
+    """The kernel launches conv_fft makes for these shapes, planned on the host
+    without a device (see ndc_conv_fft_launches): a list of
+    (dynamic LDS bytes, grid, block) per launch, for a device with `lds_bytes`
+    of dynamic LDS per workgroup and `cus` compute units."""
+    d = _desc_for_shapes(data_shape, kernel_shape, conv_mode, padding_mode, dilation, reverse_flag, dtype)
+    lib = C.lib()
+    cap = 64
+    bufs = [(ctypes.c_int64 * cap)() for _ in range(3)]
+    cnt = ctypes.c_int64()
+    _check(lib.ndc_conv_fft_launches(ctypes.byref(d), int(lds_bytes), int(cus), bufs[0], bufs[1], bufs[2], cap,
+                                     ctypes.byref(cnt)))
+    return [(bufs[0][i], bufs[1][i], bufs[2][i]) for i in range(min(cap, cnt.value))]
+
+
 _ALGOS = {"auto": C.ALGO_AUTO, "direct": C.ALGO_DIRECT, "bf16": C.ALGO_MFMA_BF16, "bf16x3": C.ALGO_MFMA_BF16X3}
 
 
@@ -642,7 +658,7 @@ def conv_fft_par(data, kernel, conv_mode, padding_mode):
 # ---------------------------------------------------------------------------
 
 _BATCH_ALGOS = {"auto": C.BATCH_AUTO, "fft": C.BATCH_FFT, "direct": C.BATCH_DIRECT, "plain": C.BATCH_PLAIN,
-                "bf16x3": C.BATCH_MFMA, "auto_bf16x3": C.BATCH_AUTO_BF16X3}
+                "bf16x3": C.BATCH_MFMA, "auto_bf16x3": C.BATCH_AUTO_BF16X3, "bf16x6": C.BATCH_BF16X6}
 
 
 def _batch_desc(x_lens, k_lens, conv_mode, padding_mode, dilation, reverse_flag, algo, host):
@@ -674,9 +690,11 @@ class BatchPlan:
     with the shared ConvMode, PaddingMode, dilation and reverse flag.  Signals,
     kernels and outputs are packed back to back in pair order.  algo: "auto"
     (a device-measured cost model picks FP32 direct or FP32 overlap-save FFT per
-    pair), "auto_bf16x3" (the same choice with BF16x3 matrix-core direct as a
-    third form), "fft", "direct" (FP32), "bf16x3", or "plain" (one
-    single-problem conv_fft per pair).
+    pair), "auto_bf16x3" (the same choice with the matrix-core direct forms:
+    BF16x3 for kernels of at least 32 taps, BF16x6 below), "fft", "direct"
+    (FP32), "bf16x3" (matrix cores for every eligible pair, BF16x3 or BF16x6 by
+    the same tap rule), "bf16x6" (the three-term split for every eligible
+    pair), or "plain" (one single-problem conv_fft per pair).
     host=True takes numpy arrays; otherwise torch GPU tensors, ordered on
     torch's current stream."""
 

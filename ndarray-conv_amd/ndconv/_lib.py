@@ -25,7 +25,7 @@ FLAG_HOST_MEMORY, FLAG_X_STRIDED, FLAG_K_STRIDED = 1, 2, 4
 # ndc_op
 OP_CONV, OP_CONV_FFT = 0, 1
 # ndc_batch_algo
-BATCH_AUTO, BATCH_FFT, BATCH_DIRECT, BATCH_PLAIN, BATCH_MFMA, BATCH_AUTO_BF16X3 = range(6)
+BATCH_AUTO, BATCH_FFT, BATCH_DIRECT, BATCH_PLAIN, BATCH_MFMA, BATCH_AUTO_BF16X3, BATCH_BF16X6 = range(7)
 
 I64A = ctypes.c_int64 * MAX_RANK
 
@@ -118,6 +118,9 @@ def lib():
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [H, ctypes.c_int32, ctypes.c_int32, I64P, P, P, ctypes.c_uint32]
+    L.ndc_conv_fft_launches.restype = ctypes.c_int
+    L.ndc_conv_fft_launches.argtypes = [ctypes.POINTER(ConvDesc), ctypes.c_int64, ctypes.c_int32, I64P, I64P, I64P,
+                                        ctypes.c_int64, I64P]
     L.ndc_conv_fft_bake.restype = ctypes.c_int
     L.ndc_conv_fft_bake.argtypes = [H, ctypes.POINTER(ConvDesc), P, ctypes.POINTER(B)]
     L.ndc_conv_fft_with_baked.restype = ctypes.c_int

@@ -53,9 +53,11 @@ def random_border(rng, cval):
     return ("const", cval) if b == "const" else b
 
 
-def random_case(rng, n, cval, max_x=9, max_k=4, allow_stride=True):
+def random_case(rng, n, cval, max_x=9, max_k=4, allow_stride=True, signed=True):
     """(xshape, kshape, dilation, reverse, conv mode, padding mode) in the
-    oracle's representation; pads may exceed the axis length."""
+    oracle's representation; pads may exceed the axis length.  Per-side
+    constants are cval and -cval (cval twice for unsigned element types)."""
+    neg = -cval if signed else cval
     xs = [int(v) for v in rng.integers(1, max_x + 1, n)]
     ks = [int(v) for v in rng.integers(1, max_k + 1, n)]
     dil = [int(v) for v in rng.integers(1, 3, n)]
@@ -87,7 +89,7 @@ def random_case(rng, n, cval, max_x=9, max_k=4, allow_stride=True):
     elif r == 5:
         pm = ("custom", [random_border(rng, cval) for _ in range(n)])
     else:
-        pm = ("explicit", [[random_border(rng, cval), random_border(rng, -cval)] for _ in range(n)])
+        pm = ("explicit", [[random_border(rng, cval), random_border(rng, neg)] for _ in range(n)])
     return xs, ks, dil, rev, mode, pm
 
 

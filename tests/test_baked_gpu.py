@@ -21,6 +21,7 @@ CASES = [
     ((10, 100, 200), (5, 11, 31), 1, "full", "zeros", np.complex64),
     ((64, 90), (7, 9), 3, ("custom", [2, 3], [2, 1]), "replicate", np.float64),
     ((33, 47, 5), (4, 6, 3), 1, "valid", "reflect", np.complex128),
+    ((5000, 64), (11, 5), 1, "same", "circular", np.float32),      # long axis 0: kept spectrum in split order
 ]
 
 

@@ -132,25 +132,35 @@ def test_complex_cases_gpu(gpu, case):
 
 # --- random cases over every mode, border, rank and dtype -----------------------------
 
-@pytest.mark.parametrize("dtype", [np.int32, np.float32, np.float64, np.complex64, np.int64, np.uint8,
-                                   np.int16, np.complex128])
-@pytest.mark.parametrize("rank", [1, 2, 3])
+# Every element type ConvExt is used with (any NumAssign + Copy, conv/mod.rs:118-127)
+# and every rank the crate's impls exist for: they need Dim<[Ix; N]>: RemoveAxis,
+# which ndarray provides for N = 1..6.
+CONV_DTYPES = [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32, np.uint64,
+               np.float32, np.float64, np.complex64, np.complex128]
+RANKS = [1, 2, 3, 4, 5, 6]
+MAX_X = {1: 8, 2: 8, 3: 5, 4: 4, 5: 3, 6: 3}
+
+
+@pytest.mark.parametrize("dtype", CONV_DTYPES)
+@pytest.mark.parametrize("rank", RANKS)
 def test_conv_random(gpu, dtype, rank):
     rng = np.random.default_rng(1000 * rank + np.dtype(dtype).num)
     cval = 3 if np.dtype(dtype).kind in "iu" else 1.5
-    for _ in range(25):
-        xs, ks, dil, rev, mode, pm = random_case(rng, rank, cval, max_x=8 if rank < 3 else 5)
+    for _ in range(25 if rank <= 3 else 12):
+        xs, ks, dil, rev, mode, pm = random_case(rng, rank, cval, max_x=MAX_X[rank], max_k=4 if rank <= 3 else 3,
+                                                 signed=np.dtype(dtype).kind != "u")
         x = rand_array(rng, xs, dtype)
         k = rand_array(rng, ks, dtype)
         check_conv(x, k, mode, pm, dil, rev)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128])
-@pytest.mark.parametrize("rank", [1, 2, 3])
+@pytest.mark.parametrize("rank", RANKS)
 def test_conv_fft_random(gpu, dtype, rank):
     rng = np.random.default_rng(2000 * rank + np.dtype(dtype).num)
-    for _ in range(25):
-        xs, ks, dil, rev, mode, pm = random_case(rng, rank, 1.5, max_x=12 if rank < 3 else 6)
+    for _ in range(25 if rank <= 3 else 12):
+        xs, ks, dil, rev, mode, pm = random_case(rng, rank, 1.5, max_x=12 if rank < 3 else MAX_X[rank] + 1,
+                                                 max_k=4 if rank <= 3 else 3)
         x = rand_array(rng, xs, dtype)
         k = rand_array(rng, ks, dtype)
         check_conv_fft(x, k, mode, pm, dil, rev)
@@ -183,6 +193,23 @@ def test_conv_fft_long_last_axis(gpu, shape, kshape):
         k = rand_array(rng, list(kshape), dt)
         check_conv_fft(x, k, "same", ("custom", ["reflect"] + ["circular"] * (len(shape) - 1)))
         check_conv_fft(x, k, ("custom", [1] * len(shape), [2] * len(shape)), "replicate", 2)
+
+
+@pytest.mark.parametrize("shape,kshape,dtype", [((5000, 200), (11, 31), np.float32),
+                                                 ((2600, 64), (7, 5), np.float64),
+                                                 ((6000, 9), (3, 4), np.complex64),
+                                                 ((3, 5200, 20), (2, 9, 3), np.float32),
+                                                 ((30000, 3), (17, 2), np.float32)])
+def test_conv_fft_long_column_axes(gpu, shape, kshape, dtype):
+    # column axes longer than one workgroup's LDS: the four-step split
+    # (forward L1-point passes with twiddles, then L2-point passes; mirrored
+    # for the inverse)
+    rng = np.random.default_rng(sum(shape))
+    x = rand_array(rng, list(shape), dtype)
+    k = rand_array(rng, list(kshape), dtype)
+    check_conv_fft(x, k, "same", ("custom", ["reflect"] + ["circular"] * (len(shape) - 1)))
+    check_conv_fft(x, k, ("custom", [1] * len(shape), [2] * len(shape)), "replicate", 2, False)
+    check_conv_fft(x, k, "full", "zeros")
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64, np.float64, np.complex128])
@@ -238,3 +265,25 @@ def test_processor_reuse(gpu):
         ref = O.conv_fft(x, k, "same", "zeros")
         got = nc.conv_fft_with_processor(x, k, nc.ConvMode.Same, nc.PaddingMode.Zeros, proc)
         assert O.relerr(got, ref)[0] < 1e-12
+
+
+def test_stream_switch_orders_calls(gpu):
+    """A device call on torch's stream followed at once by a host-memory call
+    on the handle's own stream: both use the handle's spectra, so the second
+    must not start before the first has finished with them."""
+    import torch
+    rng = np.random.default_rng(13)
+    proc = nc.get_fft_processor()
+    x = rng.standard_normal((1024, 4000)).astype(np.float32)
+    k = rng.standard_normal((9, 17)).astype(np.float32)
+    xs = rng.standard_normal((1000, 4000)).astype(np.float32)
+    ks = rng.standard_normal((5, 7)).astype(np.float32)
+    xt, kt = torch.from_numpy(x).cuda(), torch.from_numpy(k).cuda()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        yt = nc.conv_fft_with_processor(xt, kt, nc.ConvMode.Same, nc.PaddingMode.Zeros, proc)
+        ys = nc.conv_fft_with_processor(xs, ks, nc.ConvMode.Same, nc.PaddingMode.Zeros, proc)
+        torch.cuda.synchronize()
+        assert O.relerr(yt.cpu().numpy(), O.conv_fft(x.astype(np.float64), k.astype(np.float64), "same",
+                                                     "zeros"))[0] <= 1e-5
+        assert O.relerr(ys, O.conv_fft(xs.astype(np.float64), ks.astype(np.float64), "same", "zeros"))[0] <= 1e-5
