@@ -1842,10 +1842,12 @@ ndc_status run_fft(const Plan& p, const void* xv, const void* kv, void* yv, Laun
     };
 
     // A column line of F points fits one workgroup when the fused pass's
-    // three tile buffers of one column fit the LDS budget.
+    // three tile buffers of one column fit the LDS.  Narrow tiles of long
+    // lines coalesce poorly, but one pass still beats the split's three: on a
+    // 4096x32768 f32 convolution (axis-0 length 4608, one-column tiles) the
+    // fused pass took 4.3 ms against 6.4 ms for the split passes.
     auto col_fits = [&](int64_t F) {
-        return F <= (int64_t(1) << 24) &&
-               (size_t(ntw_of(int(F), 0, kDbl)) + size_t(3) * pp_len(int(F))) * es <= std::min<size_t>(lds, 96 * 1024);
+        return F <= (int64_t(1) << 24) && (size_t(ntw_of(int(F), 0, kDbl)) + size_t(3) * pp_len(int(F))) * es <= lds;
     };
     // F = L1 * L2 with both factors fitting, as close to sqrt(F) as possible
     // (0 when no such split exists)
