@@ -212,6 +212,19 @@ ndc_status ndc_conv(ndc_handle h, const ndc_conv_desc* desc, const void* x, cons
  * threading and planner reuse, which the handle subsumes. */
 ndc_status ndc_conv_fft(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
 
+/* ndc_conv_fft for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  (No
+ * counterpart in the reference: its CPU path has no device memory to manage.)
+ * Same arguments and results as ndc_conv_fft, device pointers only
+ * (NDC_FLAG_HOST_MEMORY is NDC_ERR_INVALID_ARGUMENT); the workspace (spectra
+ * and transform tables) comes from one library-owned 32 MiB device array per
+ * GPU instead of allocations, so a convolution whose workspace bound exceeds
+ * it returns NDC_ERR_UNSUPPORTED (ndc_last_error gives both sizes; about
+ * 2 x 8 x (F0 ... F(N-2)) x (F(N-1)/2 + 8) bytes for f32, so 200x5000 fits and
+ * 4096x4096 does not).  Calls that overlap on one GPU (other streams or host
+ * threads) share the array and race. */
+ndc_status ndc_conv_fft_noalloc(ndc_handle h, const ndc_conv_desc* desc, const void* x, const void* k, void* y);
+
 /* A kernel spectrum kept for repeated convolutions of same-shaped data with
  * one kernel: the reference's sketched ConvFFTExt::conv_fft_bake /
  * conv_fft_with_baked and Baked (conv_fft/mod.rs:22-42,175-182,306-392,

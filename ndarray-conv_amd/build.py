@@ -19,7 +19,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libndconv.so")
 
 SOURCES = ["plan.cpp", "capi.cpp", "batch.cpp", "direct_conv.hip", "fft_conv.hip", "mfma_conv.hip",
-           "batch_conv.hip"]
+           "batch_conv.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

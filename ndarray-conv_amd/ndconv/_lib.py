@@ -108,7 +108,7 @@ def lib():
     L.ndc_border_map.restype = ctypes.c_int
     L.ndc_border_map.argtypes = [ctypes.POINTER(ConvDesc), ctypes.c_int32, ctypes.c_int32,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
-    for name in ("ndc_conv", "ndc_conv_fft"):
+    for name in ("ndc_conv", "ndc_conv_fft", "ndc_conv_fft_noalloc"):
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = [H, ctypes.POINTER(ConvDesc), P, P, P]
