@@ -2014,6 +2014,11 @@ struct LineFftArgs {
     int32_t ntiles;      // column tiles per outer index
     int32_t nblocks;     // grid size
     R scale;
+    // first pass of a long line split as n_full = n * L2 (columns only): inner
+    // index I = j2 * tw_inner + i, and output k1 is multiplied by
+    // W^(k1 j2) of twN (W_nfull^e, e < n_full; conjugated for the inverse)
+    const Cx<R>* twN;
+    int64_t tw_inner;
     LinePlan<R> pl;      // complex length transformed: F/2 for even real lines, else n
 };
 
@@ -2062,7 +2067,14 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
         const C* res = fft_lines<R, true>(b0, b1, a.pl, tw, Ct, 1, Ct, inv, T) ? b1 : b0;
         for (int it = threadIdx.x; it < L * Ct; it += T) {
             const int j = fdiv(it, Ct, rC), c = it - j * Ct;
-            if (c < nc) a.out[base + int64_t(j) * a.inner + c] = cscale(res[pp(j) * Ct + c], a.scale);
+            if (c >= nc) continue;
+            C v = cscale(res[pp(j) * Ct + c], a.scale);
+            if (a.twN) {
+                const int64_t j2 = (i0 + c) / a.tw_inner;
+                const C w = a.twN[int64_t(j) * j2];
+                v = cmul(v, inv ? cconj(w) : w);
+            }
+            a.out[base + int64_t(j) * a.inner + c] = v;
         }
         return;
     }
@@ -2164,15 +2176,121 @@ __global__ void __launch_bounds__(256) transpose_kernel(const Cx<R>* in, Cx<R>* 
     }
 }
 
-// One line-transform launch along an axis of [outer][n][inner].
+// Long lines split as n = L1 * L2 (point j1 L2 + j2) leave spectral index
+// k1 + L1 k2 at k1 L2 + k2: per outer index an L1 x L2 matrix whose transpose
+// is natural order.  out[o][k][i] = f(in[o][k1 L2 + k2][i]) for k = k1 + L1 k2
+// < count: f scales (complex output) or takes the scaled real part.  Rows
+// (inner == 1) go through 32x32 LDS tiles of the matrix (blockIdx.z = outer).
+template <typename R>
+__global__ void __launch_bounds__(256) split_order_rows_kernel(const Cx<R>* in, Cx<R>* out, R* rout, int L1, int L2,
+                                                               int64_t count, R scale) {
+    __shared__ Cx<R> tile[32][33];
+    const int64_t o = blockIdx.z;
+    const Cx<R>* src = in + o * int64_t(L1) * L2;
+    const int k10 = blockIdx.y * 32, k20 = blockIdx.x * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int k1 = k10 + r, k2 = k20 + threadIdx.x;
+        if (k1 < L1 && k2 < L2) tile[r][threadIdx.x] = src[int64_t(k1) * L2 + k2];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int k2 = k20 + r, k1 = k10 + threadIdx.x;
+        const int64_t k = int64_t(k2) * L1 + k1;
+        if (k1 < L1 && k2 < L2 && k < count) {
+            const Cx<R> v = tile[threadIdx.x][r];
+            if (rout) rout[o * count + k] = v.x * scale;
+            else out[o * count + k] = cscale(v, scale);
+        }
+    }
+}
+
+// Columns (inner > 1): consecutive threads along the inner index.
+template <typename R>
+__global__ void __launch_bounds__(256) split_order_cols_kernel(const Cx<R>* in, Cx<R>* out, int64_t outer, int L1,
+                                                               int L2, int64_t inner, R scale) {
+    const int64_t n = int64_t(L1) * L2;
+    const int64_t total = outer * n * inner;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t i = e % inner, q = e / inner;
+        const int64_t k = q % n, o = q / n;
+        const int64_t k1 = k % L1, k2 = k / L1;
+        out[e] = cscale(in[(o * n + k1 * L2 + k2) * inner + i], scale);
+    }
+}
+
+// Complex input of a long real line: mode 1 (r2c, odd n) x[j] + 0i; mode 2
+// (c2r) the bins of the full line from the stored half: odd n completes the
+// Hermitian spectrum, even n forms the half-length inverse input
+// (A + B) + i W_n^-j (A - B) from A = X[j], B = conj X[M - j] (M = n / 2),
+// with the imaginary parts of the DC and Nyquist bins ignored.
+template <typename R>
+__global__ void __launch_bounds__(256) split_real_in_kernel(const R* rin, const Cx<R>* xin, Cx<R>* out, int64_t outer,
+                                                            int n, int mode, const Cx<R>* twn) {
+    using C = Cx<R>;
+    const int H = n / 2 + 1;
+    const bool half = mode == 2 && n % 2 == 0;
+    const int len = half ? n / 2 : n;
+    const int64_t total = outer * len;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t o = e / len;
+        const int j = int(e - o * len);
+        C v;
+        if (mode == 1) {
+            v = cmake<C>(rin[o * n + j], R(0));
+        } else if (!half) {
+            const C* X = xin + o * H;
+            v = j < H ? X[j] : cconj(X[n - j]);
+            if (j == 0) v.y = R(0);
+        } else {
+            const C* X = xin + o * H;
+            const int M = len;
+            C A = X[j], B = cconj(X[M - j]);
+            if (j == 0) {
+                A.y = R(0);
+                B.y = R(0);
+            }
+            v = c2r_pre(A, B, j, twn);
+        }
+        out[e] = v;
+    }
+}
+
+// r2c of a long even real line from its half-length spectrum Z (natural
+// order, M = n / 2 points per row): X[k], k <= M.
+template <typename R>
+__global__ void __launch_bounds__(256) split_r2c_post_kernel(const Cx<R>* Z, Cx<R>* out, int64_t outer, int M,
+                                                             const Cx<R>* twn) {
+    const int H = M + 1;
+    const int64_t total = outer * H;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t o = e / H;
+        const int k = int(e - o * H);
+        out[e] = r2c_bin(Z + o * M, M, k, twn);
+    }
+}
+
+template <typename R>
+ndc_status line_fft_long(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
+                         int64_t inner, R scale);
+
+// One line-transform launch along an axis of [outer][n][inner]; lines that do
+// not fit one workgroup's LDS take the split form (line_fft_long).  twN and
+// tw_inner: the first pass of such a split (see LineFftArgs).
 template <typename R>
 ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
-                    int64_t inner, R scale) {
+                    int64_t inner, R scale, const Cx<R>* twN = nullptr, int64_t tw_inner = 1) {
     using C = Cx<R>;
     LineFftArgs<R> a;
     std::memset(&a, 0, sizeof(a));
     const bool half = mode != 0 && n % 2 == 0;
     const int L = half ? n / 2 : n;
+    if ((size_t(ntw_of(L, half ? n : 0, sizeof(R) == 8)) + size_t(2) * pp_len(L)) * sizeof(C) > ctx.max_lds()) {
+        if (twN) {
+            ctx.err = "split line factor exceeds the LDS transform size";
+            return NDC_ERR_UNSUPPORTED;
+        }
+        return line_fft_long<R>(ctx, mode, inv, in, out, outer, n, inner, scale);
+    }
     if (!make_lineplan(&a.pl, L, half ? n : 0, ctx)) {
         ctx.err = "FFT plan failed";
         return NDC_ERR_UNSUPPORTED;
@@ -2188,6 +2306,8 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
     a.mode = mode;
     a.inv = inv ? 1 : 0;
     a.scale = scale;
+    a.twN = twN;
+    a.tw_inner = tw_inner;
     const size_t es = sizeof(C);
     const size_t lds = ctx.max_lds();
     int Ct = 1;
@@ -2216,6 +2336,104 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
         dim3(unsigned(blocks)), dim3(threads_for(int64_t(L) * Ct)), smem, a);
     ctx.kend();
     return check_launch(ctx, "processor line transform");
+}
+
+inline unsigned grid_for(int64_t total) {
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, int64_t(1) << 20)));
+}
+
+// A line of n points that does not fit one workgroup: n (the complex length
+// for c2c and odd real lines, n / 2 for even real ones) = L1 * L2 with both
+// factors LDS-resident.  Pass A transforms L1-point lines along j1 (L2 and the
+// inner index together form the inner extent) and multiplies by
+// W^(k1 j2); pass B transforms L2-point lines along j2; the result, in
+// k1 L2 + k2 order, is transposed back to natural order.  Real lines go
+// through a complex input (odd n) or the half-length packing (even n) with
+// its pre/post-processing, as in the LDS-resident kernel.
+template <typename R>
+ndc_status line_fft_long(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
+                         int64_t inner, R scale) {
+    using C = Cx<R>;
+    constexpr bool kDbl = sizeof(R) == 8;
+    const size_t es = sizeof(C);
+    const size_t lds = ctx.max_lds();
+    if (mode != 0 && inner != 1) {
+        ctx.err = "real transforms run along rows";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    const bool half = mode != 0 && n % 2 == 0;
+    const int64_t Lc = half ? n / 2 : n;
+    auto fits = [&](int64_t L) {
+        return L >= 2 && (size_t(ntw_of(int(L), 0, kDbl)) + size_t(2) * pp_len(int(L))) * es <= lds;
+    };
+    int64_t L2 = 0;
+    for (int64_t l2 = 2; l2 * l2 <= Lc; ++l2)
+        if (Lc % l2 == 0 && fits(l2) && fits(Lc / l2)) L2 = l2;
+    if (!L2) {
+        ctx.err = "line length cannot be split into LDS-resident factors";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    const int64_t L1 = Lc / L2;
+    const C* twc = static_cast<const C*>(ctx.twiddles(Lc, kDbl));
+    const C* twn = half ? static_cast<const C*>(ctx.twiddles(n, kDbl)) : twc;
+    const int64_t elems = outer * Lc * inner;
+    C* t1 = static_cast<C*>(ctx.scratch(1, size_t(elems) * es));
+    C* t2 = mode != 0 ? static_cast<C*>(ctx.scratch(2, size_t(outer * std::max<int64_t>(Lc, n / 2 + 1)) * es)) : nullptr;
+    if (!twc || !twn || !t1 || (mode != 0 && !t2)) {
+        ctx.err = "split line workspace allocation failed";
+        return NDC_ERR_OUT_OF_MEMORY;
+    }
+    ndc_status s = NDC_OK;
+    // complex input of the c2c transform
+    const C* src = static_cast<const C*>(in);
+    if (mode == 1 && !half) {
+        ctx.kbegin("fft_processor_split_io", double(outer) * n * (sizeof(R) + es), 0);
+        NDC_LAUNCH(ctx, ctx.stream(), (split_real_in_kernel<R>), dim3(grid_for(outer * n)), dim3(256), 0,
+                   static_cast<const R*>(in), static_cast<const C*>(nullptr), t2, outer, n, 1, twn);
+        ctx.kend();
+        if ((s = check_launch(ctx, "split line input")) != NDC_OK) return s;
+        src = t2;
+    } else if (mode == 2) {
+        ctx.kbegin("fft_processor_split_io", double(outer) * (n / 2 + 1 + Lc) * es, 0);
+        NDC_LAUNCH(ctx, ctx.stream(), (split_real_in_kernel<R>), dim3(grid_for(outer * Lc)), dim3(256), 0,
+                   static_cast<const R*>(nullptr), static_cast<const C*>(in), t2, outer, n, 2, twn);
+        ctx.kend();
+        if ((s = check_launch(ctx, "split line input")) != NDC_OK) return s;
+        src = t2;
+    }
+    // (an even real line's samples, read as Lc complex points, are the
+    // half-length packing itself)
+    s = line_fft<R>(ctx, 0, inv, src, t1, outer, int(L1), L2 * inner, R(1), twc, inner);
+    if (s == NDC_OK) s = line_fft<R>(ctx, 0, inv, t1, t1, outer * L1, int(L2), inner, R(1));
+    if (s != NDC_OK) return s;
+    // natural order
+    auto order = [&](C* cdst, R* rdst, int64_t count, R sc) -> ndc_status {
+        if (inner == 1) {
+            const dim3 grid(unsigned((L2 + 31) / 32), unsigned((L1 + 31) / 32), unsigned(outer));
+            ctx.kbegin("fft_processor_split_order", double(outer) * (Lc + count) * es, 0);
+            NDC_LAUNCH(ctx, ctx.stream(), (split_order_rows_kernel<R>), grid, dim3(32, 8), 0, t1, cdst, rdst, int(L1),
+                       int(L2), count, sc);
+        } else {
+            ctx.kbegin("fft_processor_split_order", 2.0 * double(elems) * es, 0);
+            NDC_LAUNCH(ctx, ctx.stream(), (split_order_cols_kernel<R>), dim3(grid_for(elems)), dim3(256), 0, t1, cdst,
+                       outer, int(L1), int(L2), inner, sc);
+        }
+        ctx.kend();
+        return check_launch(ctx, "split line reorder");
+    };
+    if (mode == 0) return order(static_cast<C*>(out), nullptr, Lc, scale);
+    if (mode == 2) {
+        // even: the complex result is the real line's sample pairs; odd: real parts
+        if (half) return order(static_cast<C*>(out), nullptr, Lc, scale);
+        return order(nullptr, static_cast<R*>(out), n, scale);
+    }
+    if (!half) return order(static_cast<C*>(out), nullptr, n / 2 + 1, scale);
+    if ((s = order(t2, nullptr, Lc, R(1))) != NDC_OK) return s;
+    ctx.kbegin("fft_processor_split_io", double(outer) * (Lc + Lc + 1) * es, 0);
+    NDC_LAUNCH(ctx, ctx.stream(), (split_r2c_post_kernel<R>), dim3(grid_for(outer * (Lc + 1))), dim3(256), 0, t2,
+               static_cast<C*>(out), outer, int(Lc), twn);
+    ctx.kend();
+    return check_launch(ctx, "split line post-processing");
 }
 
 template <typename R>

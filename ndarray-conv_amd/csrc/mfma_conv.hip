@@ -31,9 +31,11 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kXB = 32 * kWaves;     // outputs along x per workgroup
-constexpr int kMaxW = 256;           // staged row width limit (u32 entries)
-constexpr int kPer = 16 * kMaxW / kThreads;   // staged entries per thread per batch
+constexpr int kXT = 1;               // 32-output x tiles per wave (independent chains)
+constexpr int kXB = 32 * kWaves * kXT;   // outputs along x per workgroup
+constexpr int kMaxW = 256;           // staged row width limit (entries)
+constexpr int kMaxSR = 16;            // input rows staged per batch, at most
+constexpr int kPer = (kMaxSR * kMaxW + kThreads - 1) / kThreads;   // staged entries per thread per batch
 constexpr int kMaxChain = 1024;      // chain rows per pass (BY + 31 * Q)
 
 // input rows staged per batch: a multiple of Q keeps the accumulator-set
@@ -83,9 +85,20 @@ __device__ __forceinline__ bool pad_source(const MfmaArgs& a, int32_t cy, int32_
     return true;
 }
 
+// True when v and its BF16 split stay finite (NaN and infinities fail, and
+// magnitudes whose BF16 rounding would overflow).
+__device__ __forceinline__ bool finite_bf16(float v) { return __builtin_fabsf(v) <= 3.38e38f; }
+
 // D1: the subsampled column dilation is 1 (stride equal to dilation, or both
 // 1), so a K-block's 8 operand columns are consecutive staged entries and
 // their reads take fixed offsets from one address.
+//
+// Each wave runs kXT chains side by side (32 adjacent outputs each, tile xt
+// of wave w at x0 + 32 (w + kWaves xt)) sharing the B operands.  One tile per
+// wave is the measured best on the 4096x4096 BF16x3 case (76.5 us; two tiles
+// per wave took 117-136 us: wider workgroups halve the grid, and shorter
+// chains to restore it add fill steps).  Kernel columns beyond the pass hold
+// zero B entries, so the A operands need no masks (their staged columns exist).
 template <int NKB, int Q, bool X3, bool D1>
 __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     constexpr int kSR = rows_per_batch<Q>();
@@ -96,6 +109,9 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     extern __shared__ uint32_t rows_lds[];
     __shared__ int32_t s_row[kMaxChain];     // border code of chain row t
     __shared__ int32_t s_col[kMaxW];         // border code of staged column j
+    __shared__ int32_t s_exact;              // a staged value or tap is not finite: recompute exactly
+    if (threadIdx.x == 0) s_exact = 0;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int half = lane >> 5;
@@ -104,7 +120,6 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
     const int ty = blockIdx.x / a.ntx;
     const int x0 = tx * kXB;
     const int y0 = ty * a.BY;
-    const int xw = x0 + 32 * wave;           // this wave's first output column
     const int sxg = a.sx / a.gx, dxg = D1 ? 1 : a.dx / a.gx;
     const int BYe = min(a.BY, a.OY - y0);
 
@@ -117,6 +132,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
             const bool first = pr == 0 && pc == 0;
             // B operands: kernel rows r0 + (l % 32), columns c0 + 16 kb + 8 half + j
             u32x4 bh[NKB], bl[NKB];
+            bool kbad = false;
 #pragma unroll
             for (int kb = 0; kb < NKB; ++kb) {
                 uint32_t p[8];
@@ -130,6 +146,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                         const int kj = a.rev ? a.KW - 1 - (c0 + c) : c0 + c;
                         v = a.k[int64_t(ki) * a.kstr[0] + int64_t(kj) * a.kstr[1]];
                     }
+                    kbad |= !finite_bf16(v);
                     p[j] = split_pack(v);
                 }
 #pragma unroll
@@ -138,22 +155,14 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                     bl[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
                 }
             }
-            // A-column validity: kernel columns beyond this pass contribute
-            // zero; as AND masks on the packed BF16 pairs (columns 2i, 2i+1)
-            u32x4 amask[NKB];
-#pragma unroll
-            for (int kb = 0; kb < NKB; ++kb) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int c = 16 * kb + 8 * half + 2 * i;
-                    amask[kb][i] = (c < KCg ? 0x0000ffffu : 0u) | (c + 1 < KCg ? 0xffff0000u : 0u);
-                }
-            }
-            floatx16 acc[Q];
+            if (kbad) s_exact = 1;
+            floatx16 acc[Q][kXT];
 #pragma unroll
             for (int s = 0; s < Q; ++s)
 #pragma unroll
-                for (int i = 0; i < 16; ++i) acc[s][i] = 0.0f;
+                for (int xt = 0; xt < kXT; ++xt)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) acc[s][xt][i] = 0.0f;
 
             const int nsteps = BYe + (KHg - 1) * Q;
             const int nbatch = (nsteps + kSR - 1) / kSR;
@@ -196,11 +205,14 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
             const int plane = kSR * a.W;   // entries of one BF16 plane (D1)
             auto store = [&](uint32_t* dst) {
                 uint16_t* h16 = reinterpret_cast<uint16_t*>(dst);
+                bool bad = false;
 #pragma unroll
                 for (int i = 0; i < kPer; ++i) {
                     const int e = threadIdx.x + i * kThreads;
                     if (e < kSR * a.W) {
-                        const uint32_t v = split_pack((okm >> i & 1u) ? raw[i] : fill[i]);
+                        const float f = (okm >> i & 1u) ? raw[i] : fill[i];
+                        bad |= !finite_bf16(f);
+                        const uint32_t v = split_pack(f);
                         if constexpr (D1) {
                             // hi at plane 0, lo at plane 1; the shifted planes 2, 3
                             // hold entry e at e - 1
@@ -215,6 +227,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                         }
                     }
                 }
+                if (bad) s_exact = 1;
             };
             const int bufw = D1 ? 2 * plane : plane;   // 32-bit words per buffer
             // host keeps W <= kMaxW, so kSR * W <= kPer * kThreads
@@ -232,86 +245,99 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                         if (t >= tend) break;
                         const int rowoff = (t - b * kSR) * a.W;
                         const uint32_t* row = buf + rowoff;
-                        // Hankel A operands from the staged row
-                        u32x4 ah[NKB], al[NKB];
+                        // Hankel A operands from the staged row, per tile
+                        u32x4 ah[kXT][NKB], al[kXT][NKB];
 #pragma unroll
-                        for (int kb = 0; kb < NKB; ++kb) {
-                            const int base = (32 * wave + m) * sxg + (16 * kb + 8 * half) * dxg;
-                            // every staged column of the K-block exists (W covers
-                            // 16 * NKB kernel columns): the reads are unconditional
-                            // and only a partial last block selects
-                            if constexpr (D1) {
-                                // columns base..base+7 of the hi / lo plane, from the
-                                // shifted planes when base is odd: even half-word
-                                // offsets, so 32-bit reads of (column 2i, 2i+1) pairs
-                                const int odd = base & 1;
-                                const uint16_t* b16 = reinterpret_cast<const uint16_t*>(buf);
-                                const int oh = (odd ? 2 * plane : 0) + rowoff + base - odd;
-                                const uint32_t* ph = reinterpret_cast<const uint32_t*>(b16 + oh);
-                                const uint32_t* pl = reinterpret_cast<const uint32_t*>(b16 + oh + plane);
+                        for (int xt = 0; xt < kXT; ++xt) {
 #pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    ah[kb][i] = ph[i];
-                                    al[kb][i] = X3 ? pl[i] : 0u;
-                                }
-                            } else {
-                                uint32_t p[8];
+                            for (int kb = 0; kb < NKB; ++kb) {
+                                const int base = (32 * (wave + kWaves * xt) + m) * sxg + (16 * kb + 8 * half) * dxg;
+                                // every staged column of the K-block exists (W covers
+                                // 16 * NKB kernel columns): the reads are unconditional
+                                if constexpr (D1) {
+                                    // columns base..base+7 of the hi / lo plane, from the
+                                    // shifted planes when base is odd: even half-word
+                                    // offsets, so 32-bit reads of (column 2i, 2i+1) pairs
+                                    const int odd = base & 1;
+                                    const uint16_t* b16 = reinterpret_cast<const uint16_t*>(buf);
+                                    const int oh = (odd ? 2 * plane : 0) + rowoff + base - odd;
+                                    const uint32_t* ph = reinterpret_cast<const uint32_t*>(b16 + oh);
+                                    const uint32_t* pl = reinterpret_cast<const uint32_t*>(b16 + oh + plane);
 #pragma unroll
-                                for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
+                                    for (int i = 0; i < 4; ++i) {
+                                        ah[xt][kb][i] = ph[i];
+                                        al[xt][kb][i] = X3 ? pl[i] : 0u;
+                                    }
+                                } else {
+                                    uint32_t p[8];
 #pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    ah[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
-                                    al[kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
-                                }
-                            }
-                            if (16 * kb + 16 > KCg) {
+                                    for (int j = 0; j < 8; ++j) p[j] = row[base + j * dxg];
 #pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    const uint32_t mk = amask[kb][i];
-                                    ah[kb][i] &= mk;
-                                    al[kb][i] &= mk;
+                                    for (int i = 0; i < 4; ++i) {
+                                        ah[xt][kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+                                        al[xt][kb][i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+                                    }
                                 }
                             }
                         }
-                        // shift the accumulator one kernel row up; row 0 starts fresh
-                        floatx16 c;
+                        // shift the accumulators one kernel row up; row 0 starts fresh
+                        floatx16 c[kXT];
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) {
-                            // copy the element out first: a bit cast of the vector
-                            // element lvalue reads element 0
-                            const float ai = acc[s][i];
-                            const int v = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, ai), 0x138, 0xf, 0xf,
-                                                                      true);
-                            c[i] = m == 0 ? 0.0f : __builtin_bit_cast(float, v);
+                        for (int xt = 0; xt < kXT; ++xt) {
+#pragma unroll
+                            for (int i = 0; i < 16; ++i) {
+                                // copy the element out first: a bit cast of the vector
+                                // element lvalue reads element 0
+                                const float ai = acc[s][xt][i];
+                                const int v = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, ai), 0x138, 0xf,
+                                                                          0xf, true);
+                                c[xt][i] = m == 0 ? 0.0f : __builtin_bit_cast(float, v);
+                            }
                         }
+                        // products of the tiles interleaved: consecutive matrix
+                        // instructions never depend on each other
 #pragma unroll
                         for (int kb = 0; kb < NKB; ++kb) {
-                            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[kb]),
-                                                                       __builtin_bit_cast(bf16x8, bh[kb]), c, 0, 0, 0);
+#pragma unroll
+                            for (int xt = 0; xt < kXT; ++xt)
+                                c[xt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    __builtin_bit_cast(bf16x8, ah[xt][kb]), __builtin_bit_cast(bf16x8, bh[kb]), c[xt], 0,
+                                    0, 0);
                             if (X3) {
-                                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[kb]),
-                                                                           __builtin_bit_cast(bf16x8, bl[kb]), c, 0, 0, 0);
-                                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[kb]),
-                                                                           __builtin_bit_cast(bf16x8, bh[kb]), c, 0, 0, 0);
+#pragma unroll
+                                for (int xt = 0; xt < kXT; ++xt)
+                                    c[xt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                        __builtin_bit_cast(bf16x8, ah[xt][kb]), __builtin_bit_cast(bf16x8, bl[kb]),
+                                        c[xt], 0, 0, 0);
+#pragma unroll
+                                for (int xt = 0; xt < kXT; ++xt)
+                                    c[xt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                        __builtin_bit_cast(bf16x8, al[xt][kb]), __builtin_bit_cast(bf16x8, bh[kb]),
+                                        c[xt], 0, 0, 0);
                             }
                         }
-                        acc[s] = c;
+#pragma unroll
+                        for (int xt = 0; xt < kXT; ++xt) acc[s][xt] = c[xt];
                         // the lane of the last kernel row holds output row y; the
                         // common case (first pass, tile inside the row) is plain
                         // stores at fixed offsets
                         const int yo = y0 + t - (KHg - 1) * Q;
                         if (yo >= y0 && yo < y0 + BYe && m == KHg - 1) {
-                            float* yr = a.y + int64_t(yo) * a.OX + xw + 4 * half;
-                            if (first && xw + 32 <= a.OX) {
 #pragma unroll
-                                for (int i = 0; i < 16; ++i) yr[8 * (i / 4) + (i % 4)] = c[i];
-                            } else {
+                            for (int xt = 0; xt < kXT; ++xt) {
+                                const int xw = x0 + 32 * (wave + kWaves * xt);
+                                float* yr = a.y + int64_t(yo) * a.OX + xw + 4 * half;
+                                if (first && xw + 32 <= a.OX) {
 #pragma unroll
-                                for (int i = 0; i < 16; ++i) {
-                                    const int xo = xw + 8 * (i / 4) + 4 * half + (i % 4);
-                                    if (xo < a.OX) {
-                                        float* yp = yr + 8 * (i / 4) + (i % 4);
-                                        *yp = first ? c[i] : *yp + c[i];
+                                    for (int i = 0; i < 16; ++i) yr[8 * (i / 4) + (i % 4)] = c[xt][i];
+                                } else {
+#pragma unroll
+                                    for (int i = 0; i < 16; ++i) {
+                                        const int xo = xw + 8 * (i / 4) + 4 * half + (i % 4);
+                                        if (xo < a.OX) {
+                                            float* yp = yr + 8 * (i / 4) + (i % 4);
+                                            *yp = first ? c[xt][i] : *yp + c[xt][i];
+                                        }
                                     }
                                 }
                             }
@@ -321,6 +347,35 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
                 if (b + 1 < nbatch) store(rows_lds + ((b + 1) & 1) * bufw);
                 __syncthreads();
             }
+        }
+    }
+    // A non-finite value makes a matrix product non-finite even where the
+    // reference's sum skips it (zero taps, dilation gaps, neighbouring outputs
+    // of a tile).  Such workgroups recompute their outputs as the reference
+    // does: FP32, the view's taps in row-major order, zero taps skipped,
+    // separate multiply and add (conv/mod.rs:177-197, dilation/mod.rs:34-60).
+    __syncthreads();
+    if (s_exact) {
+        const int nout = BYe * kXB;
+        for (int e = threadIdx.x; e < nout; e += kThreads) {
+            const int yo = y0 + e / kXB, xo = x0 + e % kXB;
+            if (xo >= a.OX) continue;
+            float acc = 0.0f;
+            for (int r = 0; r < a.KH; ++r) {
+                const int32_t cy = a.map[0][int64_t(yo) * a.sy + int64_t(r) * a.dy];
+                const int ki = a.rev ? a.KH - 1 - r : r;
+                for (int c = 0; c < a.KW; ++c) {
+                    const int kj = a.rev ? a.KW - 1 - c : c;
+                    const float kv = a.k[int64_t(ki) * a.kstr[0] + int64_t(kj) * a.kstr[1]];
+                    if (kv == 0.0f) continue;
+                    const int32_t cx = a.map[1][int64_t(xo) * a.sx + int64_t(c) * a.dx];
+                    int64_t off;
+                    float fill;
+                    const float v = pad_source(a, cy, cx, &off, &fill) ? a.x[off] : fill;
+                    acc = __fadd_rn(acc, __fmul_rn(v, kv));
+                }
+            }
+            a.y[int64_t(yo) * a.OX + xo] = acc;
         }
     }
 }
@@ -427,7 +482,7 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     a.KC = std::min<int32_t>(a.KW, 16 * nkb);
     a.npass_c = (a.KW + a.KC - 1) / a.KC;
     a.npass_r = (a.KH + 31) / 32;
-    const int SR = Q == 3 ? 15 : 16;
+    const int SR = Q == 3 ? rows_per_batch<3>() : rows_per_batch<1>();
     // chain length: long chains amortise the (KH-1)*Q fill steps, short ones
     // give more workgroups; aim for two workgroups per CU
     a.ntx = int32_t((a.OX + kXB - 1) / kXB);

@@ -172,3 +172,32 @@ def test_uneven_batch_full_size(gpu):
         print(f"{algo}: worst normwise rel err {errs[i, j]:.3e} (pair {i}: n {n[i]}, k {k[i]}), "
               f"median {np.median(errs[:, j]):.3e}")
         assert errs[i, j] <= TOL, (algo, i, int(n[i]), int(k[i]), errs[i, j])
+
+
+@pytest.mark.parametrize("algo", ["auto_bf16x3", "fft", "direct", "bf16x3", "bf16x6"])
+def test_batch_nonfinite_samples(gpu, algo):
+    """A pair with an infinite or NaN sample: the reference's conv_fft of it
+    (conv_fft/mod.rs:185-292) transforms every sample into every bin, so its
+    whole output is NaN.  The batch computes each output from a window of the
+    padded signal, so at least every output whose window reaches the sample
+    is non-finite; and the other pairs of the batch stay exact."""
+    rng = np.random.default_rng(41)
+    n = [3000, 700, 5000, 257, 9000, 1500]
+    k = [31, 200, 3, 64, 501, 17]
+    xs = [rng.uniform(-1, 1, a).astype(np.float32) for a in n]
+    ks = [rng.uniform(-1, 1, b).astype(np.float32) for b in k]
+    bad = {0: (1200, np.inf), 2: (4000, np.nan), 4: (10, -np.inf)}
+    for i, (pos, v) in bad.items():
+        xs[i][pos] = v
+    got = nc.conv_fft_batch(xs, ks, nc.ConvMode.Same, nc.PaddingMode.Circular, algo=algo)
+    for i in range(len(n)):
+        if i in bad:
+            pos = bad[i][0]
+            kl = k[i]
+            p0 = (kl - 1) // 2 + ((kl - 1) % 2)       # Same: front pad
+            # output o reads padded positions o .. o + k - 1, i.e. samples o - p0 .. o - p0 + k - 1
+            lo, hi = max(0, pos + p0 - kl + 1), min(n[i] - 1, pos + p0)
+            assert not np.isfinite(got[i][lo:hi + 1]).any(), (algo, i)
+        else:
+            ref = O.conv_fft(xs[i].astype(np.float64), ks[i].astype(np.float64), "same", "circular")
+            assert O.relerr(got[i], ref)[0] <= TOL, (algo, i)

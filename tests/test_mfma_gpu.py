@@ -116,3 +116,35 @@ def test_mfma_refuses_unsupported(gpu):
     with pytest.raises(nc.Unsupported):   # f64
         nc.conv(x.astype(np.float64), k.astype(np.float64), nc.ConvMode.Same, nc.PaddingMode.Zeros,
                 algo="bf16x3")
+
+
+@pytest.mark.parametrize("algo", ["auto", "bf16x3", "bf16"])
+@pytest.mark.parametrize("strides,dil", [([2, 2], [2, 2]), ([1, 1], [2, 1])])
+def test_nonfinite_inputs_follow_the_offset_list(gpu, algo, strides, dil):
+    """Infinities and NaNs under zero taps, in dilation gaps and at tile
+    edges: the reference sums only the non-zero taps of its offset list
+    (dilation/mod.rs:47, conv/mod.rs:188-196), so an output is non-finite
+    exactly when a non-zero tap reaches such a sample.  The matrix-core forms
+    recompute tiles holding one exactly, so the non-finite outputs must match
+    the oracle's float32 sum position by position."""
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((300, 260)).astype(np.float32)
+    k = rng.standard_normal((7, 9)).astype(np.float32)
+    k[2, :] = 0.0
+    k[:, 4] = 0.0
+    for i, j, v in [(50, 60, np.inf), (120, 7, -np.inf), (200, 200, np.nan), (299, 259, np.inf), (0, 0, np.nan),
+                    (151, 33, np.inf), (151, 35, -np.inf)]:
+        x[i, j] = v
+    mode, pm = ("custom", [3, 4], strides), "replicate"
+    with np.errstate(invalid="ignore"):
+        ref = O.conv(x, k, mode, pm, dil, True)                           # float32, offset-list order
+        ref64 = O.conv(x, k, mode, pm, dil, True, acc_dtype=np.float64)
+    got = nc.conv(x, nc.with_dilation(k, dil), conv_mode(mode), padding_mode(pm), algo=algo)
+    assert got.shape == ref.shape
+    assert np.isfinite(ref).sum() < ref.size          # the case reaches some outputs
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    np.testing.assert_array_equal(np.isposinf(got), np.isposinf(ref))
+    np.testing.assert_array_equal(np.isneginf(got), np.isneginf(ref))
+    fin = np.isfinite(ref64)
+    tol = 1e-2 if algo == "bf16" else 1e-5
+    assert O.relerr(got[fin], ref64[fin])[0] <= tol

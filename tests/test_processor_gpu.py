@@ -31,8 +31,6 @@ def make(rng, shape, dt):
 def test_forward_backward_layout_and_values(gpu, dt):
     rng = np.random.default_rng(5)
     for shape in SHAPES:
-        if np.dtype(dt).itemsize >= 8 and np.dtype(dt) != np.dtype(np.complex64) and shape[-1] > 4096:
-            continue   # double lines of 5120 exceed one workgroup's LDS (see test_too_long_line_is_refused)
         x = make(rng, shape, dt)
         p = nc.get_fft_processor(dt)
         spec = p.forward(x)
@@ -88,7 +86,35 @@ def test_torch_device_tensors(gpu):
     assert O.relerr(back.cpu().numpy(), x)[0] <= 1e-5
 
 
-def test_too_long_line_is_refused(gpu):
-    p = nc.get_fft_processor(np.complex128)
+# Lines longer than one workgroup's LDS holds (about 9000 complex64 or 4800
+# complex128 points) are split into two LDS-resident factors; real lines go
+# through the half-length packing (even) or a complex line (odd).  A prime
+# length is transformed by the generic radix stage while it fits one
+# workgroup (4999 here).
+LONG = [((5120,), np.float64), ((256, 5120), np.float64), ((9600,), np.complex64), ((3, 12000), np.complex64),
+        ((16384,), np.complex128), ((2, 40000), np.float32), ((10125,), np.float32), ((3, 10125), np.float64),
+        ((20000, 3), np.complex64), ((6000, 4), np.float64), ((7, 4999), np.float32)]
+
+
+@pytest.mark.parametrize("shape,dt", LONG, ids=lambda v: str(v))
+def test_long_lines(gpu, shape, dt):
+    rng = np.random.default_rng(sum(shape))
+    x = make(rng, shape, dt)
+    p = nc.get_fft_processor(dt)
+    spec = p.forward(x)
+    wide = np.complex128 if np.dtype(dt).kind == "c" else np.float64
+    ref = O.processor_forward(x.astype(wide))
+    assert spec.shape == ref.shape
+    tol = TOL[np.dtype(dt)] * 10
+    assert O.relerr(spec, ref)[0] <= tol, shape
+    back = p.backward(spec)
+    assert back.shape == x.shape and back.dtype == x.dtype
+    assert O.relerr(back, x)[0] <= 5 * tol, shape
+    p.close()
+
+
+def test_long_prime_line_is_refused(gpu):
+    # a prime length beyond one workgroup's LDS has no LDS-resident split
+    p = nc.get_fft_processor(np.complex64)
     with pytest.raises(nc.Unsupported):
-        p.forward(np.zeros(16384, np.complex128))
+        p.forward(np.zeros(10007, np.complex64))
