@@ -178,6 +178,23 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
     std::vector<std::vector<int32_t>> cls_kpairs(kBatchClasses);
     std::vector<std::pair<double, BatchItem>> work;
     const int tile = direct_tile(s);
+    // a direct or matrix-core tile record with its pair's metadata
+    auto tile_item = [&](int kind, size_t i, int64_t o, int cnt) {
+        const PairShape& ps = sh[i];
+        BatchItem it{};
+        it.kind = kind;
+        it.a = int32_t(i);
+        it.b = int32_t(o);
+        it.c = cnt;
+        it.x_off = offs[i][0];
+        it.k_off = offs[i][1];
+        it.y_off = offs[i][2];
+        it.n = int32_t(ps.n);
+        it.k = int32_t(ps.k);
+        it.p0 = int32_t(ps.p0);
+        it.pds = int32_t(ps.pds);
+        return it;
+    };
     for (size_t i = 0; i < P; ++i) {
         const PairShape& ps = sh[i];
         BatchPair& bp = S.pairs[i];
@@ -199,8 +216,7 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
             S.main_flops += 2.0 * double(ps.out) * double(ps.k);
             for (int64_t o = 0; o < ps.out; o += tile) {
                 const int cnt = int(std::min<int64_t>(tile, ps.out - o));
-                BatchItem it{0, int32_t(i), int32_t(o), cnt};
-                work.emplace_back(cm.dir_a + cm.dir_b * double(cnt) * double(ps.k), it);
+                work.emplace_back(cm.dir_a + cm.dir_b * double(cnt) * double(ps.k), tile_item(0, i, o, cnt));
             }
         } else if (method[i] == kMethodMfma || method[i] == kMethodMfma6) {
             const bool six = method[i] == kMethodMfma6;
@@ -210,7 +226,7 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
             const int kind = six ? kBatchKindMfma6 : kBatchKindMfma;
             for (int64_t o = 0; o < ps.out; o += kBatchMfmaTile) {
                 const int cnt = int(std::min<int64_t>(kBatchMfmaTile, ps.out - o));
-                work.emplace_back(c, BatchItem{kind, int32_t(i), int32_t(o), cnt});
+                work.emplace_back(c, tile_item(kind, i, o, cnt));
             }
         } else {
             const int c = method[i];
@@ -221,7 +237,21 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
             cls_kpairs[c].push_back(int32_t(i));
             S.kspec_bytes += 4.0 * double(ps.k) + 8.0 * double(LC + 1);
             S.main_bytes += 8.0 * double(LC + 1);   // the kernel spectrum, read once (its blocks share it in cache)
-            for (int64_t o = 0; o < ps.out; o += bp.per) cls_lines[c].push_back(BatchLine{int32_t(i), int32_t(o)});
+            for (int64_t o = 0; o < ps.out; o += bp.per) {
+                BatchLine ln{};
+                ln.x_off = bp.x_off;
+                ln.y_off = bp.y_off;
+                ln.ks_off = bp.ks_off;
+                ln.n = bp.n;
+                ln.k = bp.k;
+                ln.p0 = bp.p0;
+                ln.pds = bp.pds;
+                ln.out = bp.out;
+                ln.per = bp.per;
+                ln.o0 = int32_t(o);
+                ln.pair = int32_t(i);
+                cls_lines[c].push_back(ln);
+            }
         }
     }
     for (int c = 0; c < kBatchClasses; ++c) {
@@ -230,7 +260,10 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
         const int32_t nl = int32_t(cls_lines[c].size());
         S.lines.insert(S.lines.end(), cls_lines[c].begin(), cls_lines[c].end());
         for (int32_t f = 0; f < nl; f += G) {
-            BatchItem it{1 + c, base + f, std::min(G, nl - f), 0};
+            BatchItem it{};
+            it.kind = 1 + c;
+            it.a = base + f;
+            it.b = std::min(G, nl - f);
             work.emplace_back(cm.fft_item[c], it);
         }
         const int32_t kb = int32_t(S.kpairs.size());

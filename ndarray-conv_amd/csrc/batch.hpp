@@ -65,18 +65,26 @@ struct BatchPair {
     int32_t pad_;
 };
 
-// One unit of the work queue (16 bytes).
+// One unit of the work queue (64 bytes).
 //   kind 0: direct tile  a = pair, b = first output, c = output count
 //   kind 1 + cls: FFT    a = first line, b = line count (<= batch_lines(cls))
 //   kind kBatchKindMfma / kBatchKindMfma6: BF16x3 / BF16x6 matrix-core tile,
 //       a = pair, b = first output, c = count
+// Direct and matrix-core tiles carry their pair's metadata, so a workgroup
+// starts an item after one load (which it prefetches during the item before).
 struct BatchItem {
     int32_t kind, a, b, c;
+    int64_t x_off, k_off, y_off;   // the pair's element offsets into x, k, y
+    int32_t n, k, p0, pds;         // the pair's signal and kernel lengths, front pad, padded length
+    int32_t pad_[2];
 };
+static_assert(sizeof(BatchItem) == 64, "work-queue records are 16 dwords");
 
-// One overlap-save block: pair and first output index.
+// One overlap-save block with its pair's metadata: first output o0, outputs
+// per block, and what the pair's kernel spectrum and signal need.
 struct BatchLine {
-    int32_t pair, o0;
+    int64_t x_off, y_off, ks_off;
+    int32_t n, k, p0, pds, out, per, o0, pair;
 };
 
 // Kernel-spectrum pre-pass work: up to batch_lines(cls) pairs of one class.

@@ -82,6 +82,26 @@ __device__ __forceinline__ void staged(int n, F&& f, S&& st) {
     }
 }
 
+// st(e, f(e), f(e + 1)) for even e < n (n even), with U pairs per thread in
+// flight before any is used.
+template <int U, typename F, typename S>
+__device__ __forceinline__ void staged_pairs(int n, F&& f, S&& st) {
+    for (int base = 2 * tid(); base < n; base += 2 * U * T) {
+        float v0[U], v1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + 2 * u * T;
+            v0[u] = e < n ? f(e) : 0.f;
+            v1[u] = e < n ? f(e + 1) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + 2 * u * T;
+            if (e < n) st(e, v0[u], v1[u]);
+        }
+    }
+}
+
 // W_L^e from the quarter table q[r] = W_L^r, r < L/4 (L a power of two).
 __device__ __forceinline__ C twl(const C* q, int e, int L) {
     e &= L - 1;
@@ -219,26 +239,33 @@ __device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, i
         LineCtx c{};
         c.valid = g < count;
         if (c.valid) {
-            int pi;
             if (kernel_lines) {
-                pi = a.kpairs[first + g];
+                const BatchPair p = a.pairs[a.kpairs[first + g]];
                 c.o0 = 0;
+                c.x_off = p.x_off;
+                c.y_off = p.y_off;
+                c.ks_off = p.ks_off;
+                c.k_off = p.k_off;
+                c.n = p.n;
+                c.k = p.k;
+                c.p0 = p.p0;
+                c.pds = p.pds;
+                c.out = p.out;
+                c.per = p.per;
             } else {
+                // a data line's record carries its pair's metadata: one load
                 const BatchLine ln = a.lines[first + g];
-                pi = ln.pair;
                 c.o0 = ln.o0;
+                c.x_off = ln.x_off;
+                c.y_off = ln.y_off;
+                c.ks_off = ln.ks_off;
+                c.n = ln.n;
+                c.k = ln.k;
+                c.p0 = ln.p0;
+                c.pds = ln.pds;
+                c.out = ln.out;
+                c.per = ln.per;
             }
-            const BatchPair p = a.pairs[pi];
-            c.x_off = p.x_off;
-            c.y_off = p.y_off;
-            c.ks_off = p.ks_off;
-            c.k_off = p.k_off;
-            c.n = p.n;
-            c.k = p.k;
-            c.p0 = p.p0;
-            c.pds = p.pds;
-            c.out = p.out;
-            c.per = p.per;
             const int64_t P0 = int64_t(c.o0) * a.s;
             c.interior = !kernel_lines && P0 >= c.p0 && P0 + span <= int64_t(c.p0) + c.n;
             c.src = c.x_off + P0 - c.p0;
@@ -369,9 +396,8 @@ __device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem&
 }
 
 // Direct tile: outputs [o0, o0 + cnt) of one pair, taps staged in chunks.
-__device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem& it, float* win, float* taps) {
-    const BatchPair p = a.pairs[it.a];
-    const int o0 = it.b, cnt = it.c;
+__device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem& p, float* win, float* taps) {
+    const int o0 = p.b, cnt = p.c;
     const int s = a.s, d = a.d;
     const int K = p.k;
     const bool fast = s == 1 && d == 1;
@@ -445,10 +471,9 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
 // t1*u1, t2*u0, t0*u2.  Three accumulators keep the chains independent and
 // the small terms apart from the leading one.
 template <int P>
-__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& it, uint16_t* lds) {
+__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p, uint16_t* lds) {
     using L = MfmaLds<P>;
-    const BatchPair p = a.pairs[it.a];
-    const int o0 = it.b, cnt = it.c;
+    const int o0 = p.b, cnt = p.c;
     const int d = a.d;
     const int kdil = (p.k - 1) * d + 1;
     const int nkb = (kdil + 31 + 15) / 16;
@@ -461,33 +486,36 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     c.n = p.n;
     c.p0 = p.p0;
     c.pds = p.pds;
-    // window: padded positions o0 + e for every sample an A operand reads
+    // window: padded positions o0 + e for every sample an A operand reads,
+    // split and written in pairs (e even; padh keeps a pair in one 32-bit word)
     const int W = kBatchMfmaTile + 16 * nkb + 32;
-    auto put = [&](int e, float v) {
-        uint16_t t[P];
-        split_terms<P>(v, t);
+    auto put = [&](int e, float v0, float v1) {
+        uint32_t t[P];
+        split_pair<P>(v0, v1, t);
 #pragma unroll
-        for (int q = 0; q < P; ++q) wp[q * L::win_slots + padh(e)] = t[q];
+        for (int q = 0; q < P; ++q) reinterpret_cast<uint32_t*>(wp + q * L::win_slots + padh(e))[0] = t[q];
     };
     if (int64_t(o0) >= c.p0 && int64_t(o0) + W <= int64_t(c.p0) + c.n) {
         const float* src = a.x + c.x_off + (int64_t(o0) - c.p0);   // all interior: no border logic
-        staged<24>(W, [&](int e) { return src[e]; }, put);
+        staged_pairs<12>(W, [&](int e) { return src[e]; }, put);
     } else {
-        staged<24>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
+        staged_pairs<12>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
     }
-    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero outside [0, kdil)
+    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero
+    // outside [0, kdil); pairs of both copies from three consecutive taps
     const int ntp = 16 * nkb + 48;
     auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
-    // one load per tap: copy 0 [e] and copy 1 [e - 1] hold the same tap
-    staged<5>(ntp + 1, [&](int e) { return tapv(e - 32); }, [&](int e, float v) {
-        uint16_t t[P];
-        split_terms<P>(v, t);
+    for (int e = 2 * tid(); e < ntp; e += 2 * T) {
+        const float t0 = tapv(e - 32), t1 = tapv(e - 31), t2 = tapv(e - 30);
+        uint32_t c0[P], c1[P];
+        split_pair<P>(t0, t1, c0);
+        split_pair<P>(t1, t2, c1);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            if (e < ntp) tp[(2 * q) * L::taps + e] = t[q];
-            if (e >= 1) tp[(2 * q + 1) * L::taps + e - 1] = t[q];
+            reinterpret_cast<uint32_t*>(tp + (2 * q) * L::taps + e)[0] = c0[q];
+            reinterpret_cast<uint32_t*>(tp + (2 * q + 1) * L::taps + e)[0] = c1[q];
         }
-    });
+    }
     __syncthreads();
     const int t = tid();
     const int lane = t & 63, wave = t >> 6;
@@ -515,16 +543,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
     floatx16 acc0, acc1, acc2;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = acc2[i] = 0.f;
-    u32x4 av[P], bv[P];
-    load(0, av, bv);
-    for (int kb = 0; kb < nkb; ++kb) {
-        u32x4 na[P], nb[P];
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-            na[s] = av[s];
-            nb[s] = bv[s];
-        }
-        if (kb + 1 < nkb) load(kb + 1, na, nb);   // next block's operands in flight
+    auto products = [&](const u32x4* av, const u32x4* bv) {
         acc0 = mfma(av[0], bv[0], acc0);
         acc1 = mfma(av[0], bv[1], acc1);
         acc2 = mfma(av[1], bv[0], acc2);
@@ -533,12 +552,19 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& i
             acc2 = mfma(av[2], bv[0], acc2);
             acc1 = mfma(av[0], bv[2], acc1);
         }
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-            av[s] = na[s];
-            bv[s] = nb[s];
-        }
+    };
+    // two operand sets in turn: the next block's reads are in flight while
+    // the current block's products issue, with no register copies between
+    u32x4 a0[P], b0[P], a1[P], b1[P];
+    load(0, a0, b0);
+    int kb = 0;
+    for (; kb + 1 < nkb; kb += 2) {
+        load(kb + 1, a1, b1);
+        products(a0, b0);
+        if (kb + 2 < nkb) load(kb + 2, a0, b0);
+        products(a1, b1);
     }
+    if (kb < nkb) products(a0, b0);
     // D lane holds D[8 (i/4) + 4 half + i%4][m]
     float* y = a.y + p.y_off + o0;
 #pragma unroll
@@ -584,32 +610,46 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
     extern __shared__ C smem[];
     __shared__ LineCtx lc[16];
     __shared__ int s_item[2];
+    __shared__ BatchItem s_rec[2];
     C* q = smem;
     C* buf = smem + kTwSlots;
     float* win = reinterpret_cast<float*>(buf);
     float* taps = win + kWinSlots;
     int cur = -1;   // class whose twiddle table is in LDS
-    // The queue head is popped one item ahead: the pop for the next item is
-    // in flight while the current one is processed.  s_item alternates
-    // between two slots so the write for the next item never overlaps the
-    // reads of the current index.
-    int next = 0;
+    // The queue runs two items ahead of the work: while item i is processed,
+    // item i+1's record (16 dwords, one per lane of the first 16 lanes) is in
+    // flight into s_rec and the head is popped for item i+2.  s_item and
+    // s_rec alternate between two slots, so no write for a later item
+    // overlaps the reads of the current one.
     if (threadIdx.x == 0) s_item[0] = atomicAdd(a.counter, 1);
+    __syncthreads();
+    {
+        const int i0 = s_item[0];
+        if (threadIdx.x < 16 && i0 < a.nitems)
+            reinterpret_cast<int32_t*>(&s_rec[0])[threadIdx.x] =
+                reinterpret_cast<const int32_t*>(a.items + i0)[threadIdx.x];
+        if (threadIdx.x == 0) s_item[1] = atomicAdd(a.counter, 1);
+    }
     __syncthreads();
     for (int iter = 0;; ++iter) {
         const int idx = s_item[iter & 1];
         if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
-        if (threadIdx.x == 0) next = atomicAdd(a.counter, 1);
-        const BatchItem it = a.items[idx];
-        if (it.kind == 0) {
+        const int nidx = s_item[(iter + 1) & 1];
+        int32_t pre = 0;
+        if (threadIdx.x < 16 && nidx < a.nitems) pre = reinterpret_cast<const int32_t*>(a.items + nidx)[threadIdx.x];
+        int next2 = 0;
+        if (threadIdx.x == 0) next2 = atomicAdd(a.counter, 1);
+        const BatchItem& it = s_rec[iter & 1];
+        const int kind = it.kind;
+        if (kind == 0) {
             direct_item(a, it, win, taps);
-        } else if (it.kind == kBatchKindMfma) {
+        } else if (kind == kBatchKindMfma) {
             mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf));
-        } else if (it.kind == kBatchKindMfma6) {
+        } else if (kind == kBatchKindMfma6) {
             mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem));
             cur = -1;   // the staging overwrote the twiddle table
         } else {
-            const int cls = it.kind - 1;
+            const int cls = kind - 1;
             if (cls != cur) {
                 stage_tw(q, a.tw + batch_tw_offset(cls), batch_lc(cls) / 2);
                 cur = cls;
@@ -623,7 +663,11 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
                 default: fft_item<4096>(a, it, q, buf, lc); break;
             }
         }
-        if (threadIdx.x == 0) s_item[(iter + 1) & 1] = next;
+        // Every item kind passes a barrier after all threads have read idx,
+        // nidx and the record, and no thread reads s_rec[(iter + 1) & 1] in this
+        // iteration, so the writes below need no barrier before them.
+        if (threadIdx.x < 16) reinterpret_cast<int32_t*>(&s_rec[(iter + 1) & 1])[threadIdx.x] = pre;
+        if (threadIdx.x == 0) s_item[iter & 1] = next2;
         __syncthreads();
     }
 }

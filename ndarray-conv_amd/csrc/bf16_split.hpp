@@ -43,6 +43,25 @@ __device__ __forceinline__ void split_terms(float f, uint16_t* t) {
     }
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// The P-term split of two values at once: t[p] holds term p of f0 in its low
+// and of f1 in its high half.  The conversion rounds to nearest even, as
+// bf16_bits does, and gfx950 converts both values in one instruction.
+template <int P>
+__device__ __forceinline__ void split_pair(float f0, float f1, uint32_t* t) {
+    float r0 = f0, r1 = f1;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const bf16x2 h = {__bf16(r0), __bf16(r1)};
+        t[p] = __builtin_bit_cast(uint32_t, h);
+        if (p + 1 < P) {
+            r0 -= __builtin_bit_cast(float, t[p] << 16);
+            r1 -= __builtin_bit_cast(float, t[p] & 0xFFFF0000u);
+        }
+    }
+}
+
 // Eight packed (hi | lo << 16) values to the two 8-element bf16 operands:
 // hi[i] holds elements 2i, 2i+1 of the hi parts, lo[i] of the lo parts.
 __device__ __forceinline__ void unpack8(const uint32_t* p, u32x4& hi, u32x4& lo) {
