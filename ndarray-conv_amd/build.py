@@ -3,7 +3,10 @@
     python ndarray-conv_amd/build.py [--jobs N]
 
 Every translation unit is compiled with hipcc for gfx950 only; packed FP32
-instructions are disabled (-packed-fp32-ops) for all device code.
+instructions are disabled (-packed-fp32-ops) for all device code.  The atomic
+optimizer is off: it rewrites a single lane's fetch-and-add (the batch
+work-queue pop) into a wave-wide broadcast that waits for the atomic's return
+at once, instead of where the popped index is first used.
 """
 import argparse
 import concurrent.futures as cf
@@ -25,6 +28,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
           "-Wall", "-Wno-unused-function", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
 

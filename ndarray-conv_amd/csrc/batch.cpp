@@ -287,9 +287,21 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
 
 // Device-resident copy of a schedule plus the launch arguments.
 struct DevSchedule {
-    DevMem pairs, items, lines, kpairs, kitems, kspec, counter;
+    DevMem pairs, items, lines, kpairs, kitems, kspec, counter, consts;
     BatchArgs args{};
     int grid = 0;
+    // Border sides of every pair, and the constant cells the kernels read for
+    // pad positions that hold a constant.
+    bool set_borders(int32_t front, int32_t back, float cfront, float cback) {
+        const float cells[4] = {0.f, cfront, cback, 0.f};
+        if (!consts.upload(cells, sizeof(cells))) return false;
+        args.consts = static_cast<const float*>(consts.p);
+        args.front = front;
+        args.back = back;
+        args.cfront = cfront;
+        args.cback = cback;
+        return true;
+    }
     bool upload(const Schedule& S, const float2* tw, int blocks) {
         if (!pairs.upload(S.pairs.data(), S.pairs.size() * sizeof(BatchPair)) ||
             !items.upload(S.items.data(), S.items.size() * sizeof(BatchItem)) ||
@@ -343,8 +355,7 @@ bool time_schedule(LaunchContext& ctx, const Schedule& S, const float* x, const 
     D.args.s = s;
     D.args.d = d;
     D.args.rev = 1;
-    D.args.front = kS1Circular;
-    D.args.back = kS1Circular;
+    if (!D.set_borders(kS1Circular, kS1Circular, 0.f, 0.f)) return false;
     hipStream_t st = ctx.stream();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
@@ -582,10 +593,8 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
         a.s = b->s;
         a.d = b->d;
         a.rev = d->reverse ? 1 : 0;
-        a.front = side1d(front.tag);
-        a.back = side1d(back.tag);
-        a.cfront = float(front.value.re);
-        a.cback = float(back.value.re);
+        if (!b->D.set_borders(side1d(front.tag), side1d(back.tag), float(front.value.re), float(back.value.re)))
+            return fail(h, NDC_ERR_OUT_OF_MEMORY, "batch metadata upload failed");
     }
     *out = b.release();
     return NDC_OK;

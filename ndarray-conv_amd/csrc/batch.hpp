@@ -105,6 +105,7 @@ struct BatchArgs {
     float2* kspec;
     int32_t* counter;              // work-queue head, reset by the pre-pass
     const float2* tw;              // W_L^r, r < L/4, of class c at batch_tw_offset(c) (L = 2 LC)
+    const float* consts;           // {0, cfront, cback, 0}: the cells constant pad positions read
     int32_t nitems, nkitems;
     int32_t s, d, rev;
     int32_t front, back;           // side tags of border1d.hpp

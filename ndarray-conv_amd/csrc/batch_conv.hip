@@ -214,16 +214,19 @@ struct LineCtx {
     int32_t interior, pad_;
 };
 
-__device__ __forceinline__ float pad_value(const BatchArgs& a, const LineCtx& c, int64_t P) {
-    if (P >= c.pds) return 0.f;
-    const int64_t r = P - c.p0;
-    if (r >= 0 && r < c.n) return a.x[c.x_off + r];
+// Address of padded position P: an element of x, or the cell of a.consts
+// holding the position's constant (zero past the padded end and for cells the
+// fill never writes).  Address arithmetic only, so that a window's loads can
+// all be issued before the first one returns: a load inside the border
+// branches makes the compiler wait for it before the next position.
+__device__ __forceinline__ const float* pad_ptr(const BatchArgs& a, const LineCtx& c, int64_t P) {
+    if (P >= c.pds) return a.consts;
     const int64_t s = resolve1d<int64_t>(P, c.n, c.p0, a.front, a.back);
-    if (s >= 0) return a.x[c.x_off + s];
-    if (s == kConstFront) return a.cfront;
-    if (s == kConstBack) return a.cback;
-    return 0.f;
+    if (s >= 0) return a.x + c.x_off + s;
+    return a.consts + (s == kConstFront ? 1 : s == kConstBack ? 2 : 0);
 }
+
+__device__ __forceinline__ float pad_value(const BatchArgs& a, const LineCtx& c, int64_t P) { return *pad_ptr(a, c, P); }
 
 // Tap t of the correlation form out[o] = sum_t pad[o s + t d] kc[t]:
 // kc[t] = k[K-1-t] for a convolution (reverse), k[t] for no_reverse.
@@ -274,27 +277,27 @@ __device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, i
     }
 }
 
-// Writes the real samples of the G lines: sample u of line g is val(c, u)
-// for a valid line, 0 otherwise.  kU loads in flight per thread.
+// Writes the real samples of the G lines: sample u of line g is *addr(c, u)
+// (addr returns a readable address for every line, valid or not).  Each
+// thread loads its L / T samples of every line, all in flight at once.
 template <int LC, typename F>
-__device__ __forceinline__ void load_reals(float* fb, const LineCtx* lc, F&& val) {
+__device__ __forceinline__ void load_reals(float* fb, const LineCtx* lc, F&& addr) {
     constexpr int G = kBatchPoints / LC;
     constexpr int L = 2 * LC;
-    constexpr int kU = 32;   // every sample of the item in one round
-    for (int base = tid(); base < G * L; base += kU * T) {
-        float v[kU];
+    constexpr int J = L / T;
+    static_assert(L % T == 0, "a line is whole rounds of the workgroup");
+    const int t = tid();
+    float v[G * J];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int idx = base + u * T;   // G * L is a multiple of kU * T
-            const LineCtx& c = lc[idx / L];
-            v[u] = c.valid ? val(c, idx % L) : 0.f;
-        }
+    for (int g = 0; g < G; ++g) {
+        const LineCtx& c = lc[g];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int idx = base + u * T;
-            fb[real_slot<LC>(idx / L, idx % L)] = v[u];
-        }
+        for (int j = 0; j < J; ++j) v[g * J + j] = *addr(c, t + j * T);
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int j = 0; j < J; ++j) fb[real_slot<LC>(g, t + j * T)] = v[g * J + j];
     __syncthreads();
 }
 
@@ -309,8 +312,9 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
     const int s = a.s;
     float* fb = reinterpret_cast<float*>(buf);
     // window of padded positions [o0 s, o0 s + L), border remap fused in
-    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) {
-        return c.interior ? a.x[c.src + u] : pad_value(a, c, int64_t(c.o0) * s + u);
+    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) -> const float* {
+        if (!c.valid) return a.consts;
+        return c.interior ? a.x + c.src + u : pad_ptr(a, c, int64_t(c.o0) * s + u);
     });
     // the kernel-spectrum bins this thread multiplies, loaded while the
     // forward transform runs
@@ -354,14 +358,29 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
     }
     __syncthreads();
     fft_lds<LC>(buf, q, true);
-    // crop and stride: output o0 + j is window sample kdil - 1 + j s
+    // crop and stride: output o0 + j is window sample kdil - 1 + j s (< L);
+    // every thread reads its L / T candidates of each line first (clamped
+    // into the line), then stores the ones that are outputs
+    constexpr int J = L / T;
+    const int t = tid();
+    float v[G * J];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const int kdil = lc[g].valid ? (lc[g].k - 1) * a.d + 1 : 1;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int u = kdil - 1 + (t + j * T) * s;
+            v[g * J + j] = fb[real_slot<LC>(g, u < L ? u : L - 1)];
+        }
+    }
+#pragma unroll
     for (int g = 0; g < G; ++g) {
         const LineCtx& c = lc[g];
-        if (!c.valid) continue;
-        const int kdil = (c.k - 1) * a.d + 1;
-        const int cnt = c.per < c.out - c.o0 ? c.per : c.out - c.o0;
+        const int cnt = !c.valid ? 0 : c.per < c.out - c.o0 ? c.per : c.out - c.o0;
         float* y = a.y + c.y_off + c.o0;
-        for (int j = tid(); j < cnt; j += T) y[j] = fb[real_slot<LC>(g, kdil - 1 + j * s)];
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+            if (t + j * T < cnt) y[t + j * T] = v[g * J + j];
     }
 }
 
@@ -376,11 +395,10 @@ __device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem&
     float* fb = reinterpret_cast<float*>(buf);
     // scattered kernel of the reference's FFT path (conv_fft/padding.rs:78-111):
     // k[i] at i d (reverse) or (K-1-i) d (no_reverse)
-    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) -> float {
-        if (u % d) return 0.f;
+    load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) -> const float* {
         const int i = u / d;
-        if (i >= c.k) return 0.f;
-        return a.k[c.k_off + (a.rev ? i : c.k - 1 - i)];
+        if (!c.valid || u % d || i >= c.k) return a.consts;
+        return a.k + c.k_off + (a.rev ? i : c.k - 1 - i);
     });
     fft_lds<LC>(buf, q, false);
     const float scale = 1.0f / float(L);   // the inverse normalisation, folded in
@@ -486,6 +504,29 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     c.n = p.n;
     c.p0 = p.p0;
     c.pds = p.pds;
+    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero
+    // outside [0, kdil); pairs of both copies from three consecutive taps.
+    // The first round's taps are read before the window, so both global
+    // reads are in flight together.
+    const int ntp = 16 * nkb + 48;
+    auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
+    auto put_taps = [&](int e, float t0, float t1, float t2) {
+        uint32_t c0[P], c1[P];
+        split_pair<P>(t0, t1, c0);
+        split_pair<P>(t1, t2, c1);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            reinterpret_cast<uint32_t*>(tp + (2 * q) * L::taps + e)[0] = c0[q];
+            reinterpret_cast<uint32_t*>(tp + (2 * q + 1) * L::taps + e)[0] = c1[q];
+        }
+    };
+    const int e0 = 2 * tid();
+    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+    if (e0 < ntp) {
+        f0 = tapv(e0 - 32);
+        f1 = tapv(e0 - 31);
+        f2 = tapv(e0 - 30);
+    }
     // window: padded positions o0 + e for every sample an A operand reads,
     // split and written in pairs (e even; padh keeps a pair in one 32-bit word)
     const int W = kBatchMfmaTile + 16 * nkb + 32;
@@ -501,21 +542,8 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     } else {
         staged_pairs<12>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
     }
-    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero
-    // outside [0, kdil); pairs of both copies from three consecutive taps
-    const int ntp = 16 * nkb + 48;
-    auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
-    for (int e = 2 * tid(); e < ntp; e += 2 * T) {
-        const float t0 = tapv(e - 32), t1 = tapv(e - 31), t2 = tapv(e - 30);
-        uint32_t c0[P], c1[P];
-        split_pair<P>(t0, t1, c0);
-        split_pair<P>(t1, t2, c1);
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            reinterpret_cast<uint32_t*>(tp + (2 * q) * L::taps + e)[0] = c0[q];
-            reinterpret_cast<uint32_t*>(tp + (2 * q + 1) * L::taps + e)[0] = c1[q];
-        }
-    }
+    if (e0 < ntp) put_taps(e0, f0, f1, f2);
+    for (int e = e0 + 2 * T; e < ntp; e += 2 * T) put_taps(e, tapv(e - 32), tapv(e - 31), tapv(e - 30));
     __syncthreads();
     const int t = tid();
     const int lane = t & 63, wave = t >> 6;
