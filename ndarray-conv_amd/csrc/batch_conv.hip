@@ -37,6 +37,24 @@ constexpr size_t kFftLdsBytes = size_t(kTwSlots + kDataSlots) * sizeof(C);
 
 __device__ __forceinline__ int padh(int e) { return e + 8 * (e >> 5); }   // 16 bytes of padding per 64
 
+// Word 3 of a raw (unswizzled, stride 0) buffer resource on gfx950: 32-bit
+// data format.  A raw buffer's accesses at or past num_records bytes read
+// zero and write nothing.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+// Raw buffer over [base, base + bytes); base and bytes are the same in every
+// lane (the item's record), taken from the first so the resource is scalar.
+// readfirstlane returns int: each half goes through uint32_t before widening,
+// or a low half with bit 31 set would sign-extend over the high half.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(float* base, int bytes) {
+    const uint64_t b = reinterpret_cast<uint64_t>(base);
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))));
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))));
+    const uint64_t u = (uint64_t(hi) << 32) | uint64_t(lo);
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(u), 0, __builtin_amdgcn_readfirstlane(bytes),
+                                             kRawBufferWord3);
+}
+
 template <int P>
 struct MfmaLds {
     static constexpr int nkb = (kBatchMfmaMaxKdil + 31 + 15) / 16;
@@ -78,26 +96,6 @@ __device__ __forceinline__ void staged(int n, F&& f, S&& st) {
         for (int u = 0; u < U; ++u) {
             const int e = base + u * T;
             if (e < n) st(e, v[u]);
-        }
-    }
-}
-
-// st(e, f(e), f(e + 1)) for even e < n (n even), with U pairs per thread in
-// flight before any is used.
-template <int U, typename F, typename S>
-__device__ __forceinline__ void staged_pairs(int n, F&& f, S&& st) {
-    for (int base = 2 * tid(); base < n; base += 2 * U * T) {
-        float v0[U], v1[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = base + 2 * u * T;
-            v0[u] = e < n ? f(e) : 0.f;
-            v1[u] = e < n ? f(e + 1) : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int e = base + 2 * u * T;
-            if (e < n) st(e, v0[u], v1[u]);
         }
     }
 }
@@ -488,62 +486,134 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
 // P = 2 (BF16x3) issues t0*u0, t0*u1, t1*u0 per block; P = 3 (BF16x6) also
 // t1*u1, t2*u0, t0*u2.  Three accumulators keep the chains independent and
 // the small terms apart from the leading one.
-template <int P>
-__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p, uint16_t* lds) {
-    using L = MfmaLds<P>;
-    const int o0 = p.b, cnt = p.c;
-    const int d = a.d;
-    const int kdil = (p.k - 1) * d + 1;
+// Address of tap q of the dilated correlation kernel kc_dil (zero cell
+// outside [0, kdil) and between dilated taps).
+__device__ __forceinline__ const float* tap_ptr(const BatchArgs& a, const BatchItem& p, int kdil, int q) {
+    if (q < 0 || q >= kdil || q % a.d) return a.consts;
+    const int t = q / a.d;
+    return a.k + p.k_off + (a.rev ? p.k - 1 - t : t);
+}
+
+// A matrix-core item's window samples and first round of taps, in registers:
+// thread t holds window pairs e = 2 (t + u T) and the taps of tap-plane pair
+// e = 2 t.  Loaded either at the item's start or during the item before it.
+struct MfmaStage {
+    static constexpr int U = (MfmaLds<3>::win + 2 * T - 1) / (2 * T);
+    float w0[U], w1[U];
+    float f0, f1, f2;
+};
+static_assert(MfmaLds<2>::win == MfmaLds<3>::win, "both splits read the same window");
+
+// Issues the loads of item p's stage (no waits).  The window covers padded
+// positions o0 + e for every sample an A operand reads.
+__device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchItem& p, MfmaStage& st) {
+    const int kdil = (p.k - 1) * a.d + 1;
     const int nkb = (kdil + 31 + 15) / 16;
-    if (kdil > kBatchMfmaMaxKdil) return;   // the planner never queues these
-    uint16_t* wp = lds;                       // window plane t at wp + t * win_slots
-    uint16_t* tp = lds + P * L::win_slots;    // tap plane t, copy c at tp + (2 t + c) * taps:
-                                              // copy 0 holds tap q at 32 + q, copy 1 at 31 + q
+    const int W = kBatchMfmaTile + 16 * nkb + 32;
+    const int64_t o0 = p.b;
     LineCtx c{};
     c.x_off = p.x_off;
     c.n = p.n;
     c.p0 = p.p0;
     c.pds = p.pds;
-    // taps: copy 0 [e] = kc_dil[e - 32], copy 1 [e] = kc_dil[e - 31], zero
-    // outside [0, kdil); pairs of both copies from three consecutive taps.
-    // The first round's taps are read before the window, so both global
-    // reads are in flight together.
+    const int t2 = 2 * tid();
+    if (o0 >= c.p0 && o0 + W <= int64_t(c.p0) + c.n) {
+        const float* src = a.x + c.x_off + (o0 - c.p0);   // all interior: no border logic
+#pragma unroll
+        for (int u = 0; u < MfmaStage::U; ++u) {
+            const int e = t2 + 2 * u * T;   // W is even: e + 1 < W whenever e < W
+            const float* pe = e < W ? src + e : a.consts;
+            st.w0[u] = pe[0];
+            st.w1[u] = pe[e < W ? 1 : 0];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < MfmaStage::U; ++u) {
+            const int e = t2 + 2 * u * T;
+            st.w0[u] = *(e < W ? pad_ptr(a, c, o0 + e) : a.consts);
+            st.w1[u] = *(e < W ? pad_ptr(a, c, o0 + e + 1) : a.consts);
+        }
+    }
+    st.f0 = *tap_ptr(a, p, kdil, t2 - 32);
+    st.f1 = *tap_ptr(a, p, kdil, t2 - 31);
+    st.f2 = *tap_ptr(a, p, kdil, t2 - 30);
+}
+
+// Splits item p's stage into the P bf16 planes in LDS: window plane t at
+// wp + t * win_slots (pairs stored as one 32-bit word; padh pads 16 bytes per
+// 64), tap plane t, copy c at tp + (2 t + c) * taps, copy 0 holding tap q at
+// 32 + q and copy 1 at 31 + q, zero outside [0, kdil).  Tap rounds past the
+// first are loaded here.  No barrier.
+template <int P>
+__device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const BatchItem& p, const MfmaStage& st,
+                                                 uint16_t* lds) {
+    using L = MfmaLds<P>;
+    uint16_t* wp = lds;
+    uint16_t* tp = lds + P * L::win_slots;
+    const int kdil = (p.k - 1) * a.d + 1;
+    const int nkb = (kdil + 31 + 15) / 16;
+    const int W = kBatchMfmaTile + 16 * nkb + 32;
     const int ntp = 16 * nkb + 48;
-    auto tapv = [&](int q) { return (q >= 0 && q < kdil && q % d == 0) ? corr_tap(a, p.k_off, p.k, q / d) : 0.f; };
-    auto put_taps = [&](int e, float t0, float t1, float t2) {
+    const int t2 = 2 * tid();
+#pragma unroll
+    for (int u = 0; u < MfmaStage::U; ++u) {
+        const int e = t2 + 2 * u * T;
+        if (e < W) {
+            uint32_t w[P];
+            split_pair<P>(st.w0[u], st.w1[u], w);
+#pragma unroll
+            for (int q = 0; q < P; ++q) reinterpret_cast<uint32_t*>(wp + q * L::win_slots + padh(e))[0] = w[q];
+        }
+    }
+    // pairs of both tap copies from three consecutive taps
+    auto put_taps = [&](int e, float t0, float t1, float t2_) {
         uint32_t c0[P], c1[P];
         split_pair<P>(t0, t1, c0);
-        split_pair<P>(t1, t2, c1);
+        split_pair<P>(t1, t2_, c1);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             reinterpret_cast<uint32_t*>(tp + (2 * q) * L::taps + e)[0] = c0[q];
             reinterpret_cast<uint32_t*>(tp + (2 * q + 1) * L::taps + e)[0] = c1[q];
         }
     };
-    const int e0 = 2 * tid();
-    float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-    if (e0 < ntp) {
-        f0 = tapv(e0 - 32);
-        f1 = tapv(e0 - 31);
-        f2 = tapv(e0 - 30);
+    if (t2 < ntp) put_taps(t2, st.f0, st.f1, st.f2);
+    for (int e = t2 + 2 * T; e < ntp; e += 2 * T)
+        put_taps(e, *tap_ptr(a, p, kdil, e - 32), *tap_ptr(a, p, kdil, e - 31), *tap_ptr(a, p, kdil, e - 30));
+}
+
+// Matrix-core tile: outputs [o0, o0 + cnt) of one pair with stride 1.  With
+// `have`, st holds this item's stage (loaded during the item before).  When
+// `next` is a matrix-core item, its stage is loaded into st during this
+// item's products and `have` set.  `advance` issues the work queue's own
+// reads for the items ahead after those loads: the wait for the queue's
+// reads at the item's end then covers loads issued before the products, and
+// never the item's own output stores.
+template <int P, typename Adv>
+__device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p, uint16_t* lds, MfmaStage& st,
+                                          bool& have, const BatchItem* next, Adv&& advance) {
+    using L = MfmaLds<P>;
+    const int o0 = p.b, cnt = p.c;
+    const int kdil = (p.k - 1) * a.d + 1;
+    const int nkb = (kdil + 31 + 15) / 16;
+    uint16_t* wp = lds;
+    uint16_t* tp = lds + P * L::win_slots;
+    if (kdil > kBatchMfmaMaxKdil) {   // the planner never queues these
+        advance();
+        have = false;
+        __syncthreads();
+        return;
     }
-    // window: padded positions o0 + e for every sample an A operand reads,
-    // split and written in pairs (e even; padh keeps a pair in one 32-bit word)
-    const int W = kBatchMfmaTile + 16 * nkb + 32;
-    auto put = [&](int e, float v0, float v1) {
-        uint32_t t[P];
-        split_pair<P>(v0, v1, t);
-#pragma unroll
-        for (int q = 0; q < P; ++q) reinterpret_cast<uint32_t*>(wp + q * L::win_slots + padh(e))[0] = t[q];
-    };
-    if (int64_t(o0) >= c.p0 && int64_t(o0) + W <= int64_t(c.p0) + c.n) {
-        const float* src = a.x + c.x_off + (int64_t(o0) - c.p0);   // all interior: no border logic
-        staged_pairs<12>(W, [&](int e) { return src[e]; }, put);
+    // two copies of the split: after fresh loads it waits for all of them,
+    // after a prefetch only for loads older than the last item's stores
+    if (have) {
+        mfma_stage_store<P>(a, p, st, lds);
     } else {
-        staged_pairs<12>(W, [&](int e) { return pad_value(a, c, int64_t(o0) + e); }, put);
+        mfma_stage_load(a, p, st);
+        mfma_stage_store<P>(a, p, st, lds);
     }
-    if (e0 < ntp) put_taps(e0, f0, f1, f2);
-    for (int e = e0 + 2 * T; e < ntp; e += 2 * T) put_taps(e, tapv(e - 32), tapv(e - 31), tapv(e - 30));
+    have = next != nullptr;
+    if (have) mfma_stage_load(a, *next, st);
+    advance();
     __syncthreads();
     const int t = tid();
     const int lane = t & 63, wave = t >> 6;
@@ -593,13 +663,17 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
         products(a1, b1);
     }
     if (kb < nkb) products(a0, b0);
-    // D lane holds D[8 (i/4) + 4 half + i%4][m]
-    float* y = a.y + p.y_off + o0;
+    // D lane holds D[8 (i/4) + 4 half + i%4][m].  Every wave issues all 16
+    // stores: the tile's outputs are a raw buffer of cnt floats, and the
+    // hardware drops the stores of lanes past its end.  With no store
+    // skipped, the next item's wait for its prefetched stage can let these
+    // stores run on instead of waiting for all of them.
+    const __amdgpu_buffer_rsrc_t yr = raw_buffer(a.y + p.y_off + o0, cnt * int(sizeof(float)));
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int o = wb + 32 * (8 * (i / 4) + 4 * half + (i % 4)) + m;
         const float v = acc0[i] + (acc1[i] + acc2[i]);
-        if (o < cnt) y[o] = v;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), yr, o * int(sizeof(float)), 0, 0);
     }
 }
 
@@ -637,66 +711,87 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a)
 __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs a) {
     extern __shared__ C smem[];
     __shared__ LineCtx lc[16];
-    __shared__ int s_item[2];
-    __shared__ BatchItem s_rec[2];
+    __shared__ int s_item[3];
+    __shared__ BatchItem s_rec[3];
     C* q = smem;
     C* buf = smem + kTwSlots;
     float* win = reinterpret_cast<float*>(buf);
     float* taps = win + kWinSlots;
     int cur = -1;   // class whose twiddle table is in LDS
-    // The queue runs two items ahead of the work: while item i is processed,
-    // item i+1's record (16 dwords, one per lane of the first 16 lanes) is in
-    // flight into s_rec and the head is popped for item i+2.  s_item and
-    // s_rec alternate between two slots, so no write for a later item
-    // overlaps the reads of the current one.
-    if (threadIdx.x == 0) s_item[0] = atomicAdd(a.counter, 1);
-    __syncthreads();
-    {
-        const int i0 = s_item[0];
-        if (threadIdx.x < 16 && i0 < a.nitems)
-            reinterpret_cast<int32_t*>(&s_rec[0])[threadIdx.x] =
-                reinterpret_cast<const int32_t*>(a.items + i0)[threadIdx.x];
-        if (threadIdx.x == 0) s_item[1] = atomicAdd(a.counter, 1);
+    // The queue runs three items ahead of the work.  While item i (ring slot
+    // r0) is processed, the records of items i and i+1 are in s_rec[r0] and
+    // s_rec[r1], item i+2's record (16 dwords, one per lane of the first 16
+    // lanes) is in flight, and the head is popped for item i+3.  Writes for
+    // later items go to slots no thread reads in the same iteration.
+    if (threadIdx.x == 0) {
+        s_item[0] = atomicAdd(a.counter, 1);
+        s_item[1] = atomicAdd(a.counter, 1);
+        s_item[2] = atomicAdd(a.counter, 1);
     }
     __syncthreads();
-    for (int iter = 0;; ++iter) {
-        const int idx = s_item[iter & 1];
+    {
+        const int r = threadIdx.x >> 4, i = s_item[r & 1];
+        if (threadIdx.x < 32 && i < a.nitems)
+            reinterpret_cast<int32_t*>(&s_rec[r])[threadIdx.x & 15] =
+                reinterpret_cast<const int32_t*>(a.items + i)[threadIdx.x & 15];
+    }
+    __syncthreads();
+    MfmaStage st;
+    bool have = false;   // st holds the current item's stage
+    for (int r0 = 0;;) {
+        const int r1 = r0 == 2 ? 0 : r0 + 1, r2 = r1 == 2 ? 0 : r1 + 1;
+        const int idx = s_item[r0];
         if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
-        const int nidx = s_item[(iter + 1) & 1];
         int32_t pre = 0;
-        if (threadIdx.x < 16 && nidx < a.nitems) pre = reinterpret_cast<const int32_t*>(a.items + nidx)[threadIdx.x];
-        int next2 = 0;
-        if (threadIdx.x == 0) next2 = atomicAdd(a.counter, 1);
-        const BatchItem& it = s_rec[iter & 1];
+        int next3 = 0;
+        // the queue's reads for items i+2 and i+3
+        auto advance = [&]() {
+            const int i2 = s_item[r2];
+            if (threadIdx.x < 16 && i2 < a.nitems)
+                pre = reinterpret_cast<const int32_t*>(a.items + i2)[threadIdx.x];
+            if (threadIdx.x == 0) next3 = atomicAdd(a.counter, 1);
+        };
+        const BatchItem& it = s_rec[r0];
         const int kind = it.kind;
-        if (kind == 0) {
-            direct_item(a, it, win, taps);
-        } else if (kind == kBatchKindMfma) {
-            mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf));
+        const bool mfma_next = s_item[r1] < a.nitems &&
+                               (s_rec[r1].kind == kBatchKindMfma || s_rec[r1].kind == kBatchKindMfma6);
+        const BatchItem* next = mfma_next ? &s_rec[r1] : nullptr;
+        if (kind == kBatchKindMfma) {
+            mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf), st, have, next, advance);
         } else if (kind == kBatchKindMfma6) {
-            mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem));
+            mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem), st, have, next, advance);
             cur = -1;   // the staging overwrote the twiddle table
         } else {
-            const int cls = kind - 1;
-            if (cls != cur) {
-                stage_tw(q, a.tw + batch_tw_offset(cls), batch_lc(cls) / 2);
-                cur = cls;
-                __syncthreads();
-            }
-            switch (cls) {
-                case 0: fft_item<256>(a, it, q, buf, lc); break;
-                case 1: fft_item<512>(a, it, q, buf, lc); break;
-                case 2: fft_item<1024>(a, it, q, buf, lc); break;
-                case 3: fft_item<2048>(a, it, q, buf, lc); break;
-                default: fft_item<4096>(a, it, q, buf, lc); break;
+            advance();
+            have = false;
+#pragma unroll
+            for (int u = 0; u < MfmaStage::U; ++u) st.w0[u] = st.w1[u] = 0.f;   // st is dead here
+            st.f0 = st.f1 = st.f2 = 0.f;
+            if (kind == 0) {
+                direct_item(a, it, win, taps);
+            } else {
+                const int cls = kind - 1;
+                if (cls != cur) {
+                    stage_tw(q, a.tw + batch_tw_offset(cls), batch_lc(cls) / 2);
+                    cur = cls;
+                    __syncthreads();
+                }
+                switch (cls) {
+                    case 0: fft_item<256>(a, it, q, buf, lc); break;
+                    case 1: fft_item<512>(a, it, q, buf, lc); break;
+                    case 2: fft_item<1024>(a, it, q, buf, lc); break;
+                    case 3: fft_item<2048>(a, it, q, buf, lc); break;
+                    default: fft_item<4096>(a, it, q, buf, lc); break;
+                }
             }
         }
-        // Every item kind passes a barrier after all threads have read idx,
-        // nidx and the record, and no thread reads s_rec[(iter + 1) & 1] in this
-        // iteration, so the writes below need no barrier before them.
-        if (threadIdx.x < 16) reinterpret_cast<int32_t*>(&s_rec[(iter + 1) & 1])[threadIdx.x] = pre;
-        if (threadIdx.x == 0) s_item[iter & 1] = next2;
+        // Every item kind passes a barrier after all threads have read the
+        // ring at the top, and no thread reads s_rec[r2] or writes s_item[r0]
+        // before that, so the writes below need no barrier before them.
+        if (threadIdx.x < 16) reinterpret_cast<int32_t*>(&s_rec[r2])[threadIdx.x] = pre;
+        if (threadIdx.x == 0) s_item[r0] = next3;
         __syncthreads();
+        r0 = r1;
     }
 }
 
