@@ -308,7 +308,7 @@ struct DevSchedule {
             !lines.upload(S.lines.data(), S.lines.size() * sizeof(BatchLine)) ||
             !kpairs.upload(S.kpairs.data(), S.kpairs.size() * sizeof(int32_t)) ||
             !kitems.upload(S.kitems.data(), S.kitems.size() * sizeof(BatchKItem)) ||
-            !kspec.ensure(size_t(S.ks_total) * sizeof(float2)) || !counter.ensure(256))
+            !kspec.ensure(size_t(S.ks_total) * sizeof(float2)) || !counter.ensure(size_t(kBatchQueues) * kBatchQueueStride * sizeof(int32_t)))
             return false;
         args.pairs = static_cast<const BatchPair*>(pairs.p);
         args.items = static_cast<const BatchItem*>(items.p);

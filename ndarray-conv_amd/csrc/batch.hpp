@@ -47,6 +47,13 @@ constexpr int kBatchKindMfma6 = 2 + kBatchClasses;   // BF16x6 matrix-core tile 
 // within 5 * 2^-24.
 constexpr int kBf16x3MinTaps = 32;
 
+// The work queue is kBatchQueues sub-queues: sub-queue q holds items q,
+// q + kBatchQueues, q + 2 kBatchQueues, ... of the longest-first item list.
+// One head per sub-queue, kBatchQueueStride ints (256 bytes) apart: a single
+// head shared by every workgroup serializes the pops (about 12 ns each).
+constexpr int kBatchQueues = 8;
+constexpr int kBatchQueueStride = 64;
+
 constexpr int batch_lc(int cls) { return 256 << cls; }
 constexpr int batch_lines(int cls) { return kBatchPoints / batch_lc(cls); }
 // Quarter-circle twiddle tables of all classes, concatenated: class c has
@@ -103,7 +110,7 @@ struct BatchArgs {
     const int32_t* kpairs;         // pairs of the kernel-spectrum pre-pass, grouped by class
     const BatchKItem* kitems;
     float2* kspec;
-    int32_t* counter;              // work-queue head, reset by the pre-pass
+    int32_t* counter;              // sub-queue heads (kBatchQueueStride apart), reset by the pre-pass
     const float2* tw;              // W_L^r, r < L/4, of class c at batch_tw_offset(c) (L = 2 LC)
     const float* consts;           // {0, cfront, cback, 0}: the cells constant pad positions read
     int32_t nitems, nkitems;

@@ -689,7 +689,7 @@ __device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
 __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a) {
     extern __shared__ C smem[];
     __shared__ LineCtx lc[16];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = 0;   // work-queue head for the main launch
+    if (blockIdx.x == 0 && threadIdx.x < kBatchQueues) a.counter[threadIdx.x * kBatchQueueStride] = 0;   // sub-queue heads
     if (int(blockIdx.x) >= a.nkitems) return;
     const BatchKItem it = a.kitems[blockIdx.x];
     C* q = smem;
@@ -721,12 +721,29 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
     // The queue runs three items ahead of the work.  While item i (ring slot
     // r0) is processed, the records of items i and i+1 are in s_rec[r0] and
     // s_rec[r1], item i+2's record (16 dwords, one per lane of the first 16
-    // lanes) is in flight, and the head is popped for item i+3.  Writes for
+    // lanes) is in flight, and a ticket is drawn for item i+3.  Writes for
     // later items go to slots no thread reads in the same iteration.
+    //
+    // Thread 0 draws tickets from sub-queue cq, starting with blockIdx.x mod
+    // kBatchQueues; ticket t of sub-queue q is item q + t kBatchQueues.  An
+    // empty sub-queue stays empty, so once cq is empty thread 0 moves on
+    // through the others in turn, and an index >= nitems after visiting all
+    // of them means every sub-queue is empty.
+    int cq = blockIdx.x % kBatchQueues, moved = 0;
+    auto ticket = [&]() { return atomicAdd(a.counter + cq * kBatchQueueStride, 1); };
+    auto resolve = [&](int t) {
+        int g = cq + t * kBatchQueues;
+        while (g >= a.nitems && moved < kBatchQueues - 1) {
+            cq = cq + 1 == kBatchQueues ? 0 : cq + 1;
+            ++moved;
+            g = cq + ticket() * kBatchQueues;
+        }
+        return g;
+    };
     if (threadIdx.x == 0) {
-        s_item[0] = atomicAdd(a.counter, 1);
-        s_item[1] = atomicAdd(a.counter, 1);
-        s_item[2] = atomicAdd(a.counter, 1);
+        s_item[0] = resolve(ticket());
+        s_item[1] = resolve(ticket());
+        s_item[2] = resolve(ticket());
     }
     __syncthreads();
     {
@@ -744,12 +761,13 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
         int32_t pre = 0;
         int next3 = 0;
-        // the queue's reads for items i+2 and i+3
+        // the queue's reads for items i+2 and i+3 (the ticket is resolved at
+        // the item's end)
         auto advance = [&]() {
             const int i2 = s_item[r2];
             if (threadIdx.x < 16 && i2 < a.nitems)
                 pre = reinterpret_cast<const int32_t*>(a.items + i2)[threadIdx.x];
-            if (threadIdx.x == 0) next3 = atomicAdd(a.counter, 1);
+            if (threadIdx.x == 0) next3 = ticket();
         };
         const BatchItem& it = s_rec[r0];
         const int kind = it.kind;
@@ -789,7 +807,7 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         // ring at the top, and no thread reads s_rec[r2] or writes s_item[r0]
         // before that, so the writes below need no barrier before them.
         if (threadIdx.x < 16) reinterpret_cast<int32_t*>(&s_rec[r2])[threadIdx.x] = pre;
-        if (threadIdx.x == 0) s_item[r0] = next3;
+        if (threadIdx.x == 0) s_item[r0] = resolve(next3);
         __syncthreads();
         r0 = r1;
     }
