@@ -759,14 +759,18 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         const int r1 = r0 == 2 ? 0 : r0 + 1, r2 = r1 == 2 ? 0 : r1 + 1;
         const int idx = s_item[r0];
         if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
-        int32_t pre = 0;
-        int next3 = 0;
         // the queue's reads for items i+2 and i+3 (the ticket is resolved at
-        // the item's end)
+        // the item's end).  Neither register is initialized: a register write
+        // over one that a load may still be filling makes the compiler wait
+        // for every load of the wave (the prefetched stage included).  Every
+        // lane loads pre (dword tid mod 16 of item i+2's record, or of item 0
+        // past the end of the queue), so pre is only ever written by loads,
+        // whose returns are ordered; thread 0 draws next3.
+        int32_t pre;
+        int next3;
         auto advance = [&]() {
             const int i2 = s_item[r2];
-            if (threadIdx.x < 16 && i2 < a.nitems)
-                pre = reinterpret_cast<const int32_t*>(a.items + i2)[threadIdx.x];
+            pre = reinterpret_cast<const int32_t*>(a.items + (i2 < a.nitems ? i2 : 0))[threadIdx.x & 15];
             if (threadIdx.x == 0) next3 = ticket();
         };
         const BatchItem& it = s_rec[r0];
