@@ -110,6 +110,11 @@ def make_inputs(w):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+def note(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` in `workload`, measured with rocprofv3
     --pmc FETCH_SIZE and WRITE_SIZE and scaled by what the same counters read
@@ -305,6 +310,7 @@ def run_batch(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, time.perf_counter() - t
 
+    note(f"batch plan ready ({plan_s:.1f} s); timing {args.steps} graph and eager steps")
     ms_graph, wall = timed(bp, True, args.steps, args.warmup)
     ms_eager, _ = timed(bp, False, args.steps, args.warmup)
     outputs = int(bp.y_total)
@@ -334,6 +340,7 @@ def run_batch(args):
     ys, names = [y_head], [args.batch_algo]
     fp32 = None
     if args.batch_algo != "auto":
+        note("timing the FP32-only form")
         bf = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo="auto", processor=proc)
         fp32, _ = timed(bf, True, args.steps, args.warmup)
         ys.append(yt.cpu().numpy())
@@ -343,14 +350,17 @@ def run_batch(args):
     # the unbatched path: one single-problem conv_fft launch sequence per pair
     plain = None
     if args.plain_steps > 0:
+        note(f"timing {args.plain_steps} steps of the per-pair path")
         bq = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo="plain", processor=proc)
         plain, _ = timed(bq, False, args.plain_steps, 1)
         bq.close()
 
     _, _, threads = host_cpu()
+    note(f"checking every pair against the FP64 oracle ({threads} threads)")
     errs = dict(zip(names, batch_oracle_errors(n, k, x, kv, ys, bp.y_lens, threads)))
     cpu = None
     if not args.no_cpu:
+        note("CPU baseline, all threads then one")
         cpu = batch_cpu_baseline(n, k, x, kv, threads)
         cpu["single_thread"] = batch_cpu_baseline(n, k, x, kv, 1)
     value = outputs / (ms_graph * 1e-3)
@@ -507,6 +517,7 @@ def main():
         nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
 
     y_host = yt.cpu().numpy()
+    note("checking against the oracle")
     ref = oracle_result(w, x, k)
     if np.dtype(w["dtype"]).kind in "iu":
         acc = {"bit_exact": bool(np.array_equal(y_host, ref))}
@@ -514,6 +525,8 @@ def main():
         from oracle import ndconv_oracle as O
         r, a = O.relerr(y_host, ref)
         acc = {"max_rel_err": r, "max_abs_err": a}
+    if not args.no_cpu:
+        note("CPU baseline")
     cpu = None if args.no_cpu else cpu_baseline(w, x, k, ref if np.dtype(w["dtype"]).kind not in "iu" else None)
     line = {"metric": METRIC, "value": value, "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
