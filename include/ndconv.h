@@ -109,7 +109,9 @@ typedef enum ndc_algo {
     NDC_ALGO_MFMA_BF16 = 2,     /* f32 only, rank 2: one BF16 product per tap on the
                                    matrix cores, FP32 accumulation */
     NDC_ALGO_MFMA_BF16X3 = 3    /* f32 only, rank 2: split hi/lo BF16, three products per
-                                   tap, FP32 accumulation (about 16 mantissa bits per product) */
+                                   tap, FP32 accumulation (about 16 mantissa bits per product);
+                                   kernels of fewer than 32 taps, whose outputs would keep that
+                                   error, run the exact FP32 direct kernel instead */
 } ndc_algo;
 
 enum {

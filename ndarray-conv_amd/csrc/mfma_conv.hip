@@ -37,6 +37,7 @@ constexpr int kMaxW = 256;           // staged row width limit (entries)
 constexpr int kMaxSR = 16;            // input rows staged per batch, at most
 constexpr int kPer = (kMaxSR * kMaxW + kThreads - 1) / kThreads;   // staged entries per thread per batch
 constexpr int kMaxChain = 1024;      // chain rows per pass (BY + 31 * Q)
+constexpr int kMfmaX3MinTaps = 32;   // fewest kernel taps BF16x3 is used for
 
 // input rows staged per batch: a multiple of Q keeps the accumulator-set
 // rotation aligned with the batches
@@ -432,6 +433,16 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
         return NDC_ERR_UNSUPPORTED;
     }
     if (p.out[0] > (int64_t(1) << 30) || p.out[1] > (int64_t(1) << 30)) return NDC_ERR_UNSUPPORTED;
+    // BF16x3 products keep about 16 bits (each within 3 * 2^-16 of the exact
+    // product); an output summed from fewer than 32 of them keeps that error
+    // above the f32 tolerance, and the exact FP32 kernel costs little at that
+    // size, so such kernels run it instead (shapes the matrix-core forms
+    // refuse are still refused).
+    const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
+    if (x3 && p.k_shape[0] * p.k_shape[1] < kMfmaX3MinTaps) {
+        const int64_t g = gcd64(sx, dx);
+        if (((kXB - 1) * sx + 15 * dx) / g + 1 <= kMaxW) return launch_direct(p, x, k, y, ctx);
+    }
     int64_t offs[kMaxRank + 1];
     const int32_t* maps = ctx.border_maps(p, offs);
     if (!maps) return NDC_ERR_OUT_OF_MEMORY;
@@ -499,7 +510,6 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     // (D1) or one 32-bit (hi, lo) word
     const size_t smem = size_t(2) * SR * a.W * (d1 ? 4 * sizeof(uint16_t) : sizeof(uint32_t));
     void (*fn)(const MfmaArgs&, unsigned, size_t, LaunchContext&) = nullptr;
-    const bool x3 = p.algo == NDC_ALGO_MFMA_BF16X3;
     const void* kp = x3 ? pick<true>(nkb, Q, d1, &fn) : pick<false>(nkb, Q, d1, &fn);
     if (!kp || !fn) return NDC_ERR_UNSUPPORTED;
     if (!lds_fits(ctx, kp, smem)) return NDC_ERR_UNSUPPORTED;

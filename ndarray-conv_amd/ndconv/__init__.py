@@ -549,7 +549,8 @@ def _run(op, data, kernel, conv_mode, padding_mode, processor=None, algo="auto")
 
 def conv(data, kernel, conv_mode, padding_mode, algo="auto", processor=None):
     """ConvExt::conv (conv/mod.rs:110-115).  algo: "auto"/"direct" (exact, in
-    the element type), "bf16" or "bf16x3" (f32 rank 2 on the matrix cores)."""
+    the element type), "bf16" or "bf16x3" (f32 rank 2 on the matrix cores;
+    "bf16x3" runs kernels of fewer than 32 taps exactly, in FP32)."""
     return _run(C.OP_CONV, data, kernel, conv_mode, padding_mode, processor, algo)
 
 

@@ -105,6 +105,39 @@ def test_mfma_workload4(gpu):
     assert errs["bf16"] <= 1e-2
 
 
+@pytest.mark.parametrize("algo", ["bf16x3", "bf16"])
+def test_wide_row_cases(gpu, algo):
+    """Output rows 512 to 2300 wide (many x tiles per row, ragged last tile)
+    with the column stride equal to the column dilation and the row stride
+    equal to the row dilation: kernel widths 1..97 (one to four column passes,
+    and below 32 taps the exact FP32 kernel that BF16x3 hands over to), every
+    border and both kernel orientations."""
+    rng = np.random.default_rng(515)
+    worst = 0.0
+    for i in range(14):
+        s = int(rng.integers(1, 3))
+        kh, kw = int(rng.integers(1, 14)), [1, 2, 17, 31, 33, 49, 65, 81, 97, 7, 40, 3, 64, 97][i]
+        ox = int(rng.integers(512, 2300))
+        oy = int(rng.integers(1, 30))
+        # valid-mode input of that output, plus random explicit pads
+        py = [int(rng.integers(0, kh * s + 1)), int(rng.integers(0, kh * s + 1))]
+        px = [int(rng.integers(0, kw * s + 1)), int(rng.integers(0, kw * s + 1))]
+        xs = max(1, (ox - 1) * s + (kw - 1) * s + 1 - px[0] - px[1])
+        ys = max(1, (oy - 1) * s + (kh - 1) * s + 1 - py[0] - py[1])
+        mode = ("explicit", [py, px], [s, s])
+        pm = ("explicit", [[random_border(rng, 0.5), random_border(rng, -0.25)] for _ in range(2)])
+        x = rng.standard_normal((ys, xs)).astype(np.float32)
+        k = rng.standard_normal((kh, kw)).astype(np.float32)
+        rev = bool(rng.integers(0, 2))
+        try:
+            r = run(x, k, mode, pm, [s, s], rev, algo)
+        except O.OracleError:
+            continue
+        assert r <= TOL[algo], (i, x.shape, k.shape, s, r)
+        worst = max(worst, r)
+    print(f"{algo} wide rows: worst normwise rel err {worst:.3e}")
+
+
 def test_mfma_refuses_unsupported(gpu):
     x = np.ones((20, 20), np.float32)
     k = np.ones((3, 3), np.float32)
@@ -119,8 +152,9 @@ def test_mfma_refuses_unsupported(gpu):
 
 
 @pytest.mark.parametrize("algo", ["auto", "bf16x3", "bf16"])
-@pytest.mark.parametrize("strides,dil", [([2, 2], [2, 2]), ([1, 1], [2, 1])])
-def test_nonfinite_inputs_follow_the_offset_list(gpu, algo, strides, dil):
+@pytest.mark.parametrize("strides,dil,width", [([2, 2], [2, 2], 260), ([1, 1], [2, 1], 260), ([1, 1], [1, 1], 1100),
+                                               ([2, 2], [2, 2], 1300)])
+def test_nonfinite_inputs_follow_the_offset_list(gpu, algo, strides, dil, width):
     """Infinities and NaNs under zero taps, in dilation gaps and at tile
     edges: the reference sums only the non-zero taps of its offset list
     (dilation/mod.rs:47, conv/mod.rs:188-196), so an output is non-finite
@@ -128,11 +162,11 @@ def test_nonfinite_inputs_follow_the_offset_list(gpu, algo, strides, dil):
     recompute tiles holding one exactly, so the non-finite outputs must match
     the oracle's float32 sum position by position."""
     rng = np.random.default_rng(21)
-    x = rng.standard_normal((300, 260)).astype(np.float32)
+    x = rng.standard_normal((300, width)).astype(np.float32)   # 1100 and 1300 wide: many x tiles
     k = rng.standard_normal((7, 9)).astype(np.float32)
     k[2, :] = 0.0
     k[:, 4] = 0.0
-    for i, j, v in [(50, 60, np.inf), (120, 7, -np.inf), (200, 200, np.nan), (299, 259, np.inf), (0, 0, np.nan),
+    for i, j, v in [(50, 60, np.inf), (120, 7, -np.inf), (200, 200, np.nan), (299, width - 1, np.inf), (0, 0, np.nan),
                     (151, 33, np.inf), (151, 35, -np.inf)]:
         x[i, j] = v
     mode, pm = ("custom", [3, 4], strides), "replicate"
