@@ -42,6 +42,11 @@ __device__ __forceinline__ int padh(int e) { return e + 8 * (e >> 5); }   // 16 
 // zero and write nothing.
 constexpr int kRawBufferWord3 = 0x00020000;
 
+// s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt
+// 11:8 left at their maxima, which wait for nothing).
+constexpr int kWaitVmcnt0 = 0x0F70;
+constexpr int kWaitVmcnt16 = 0x4F70;
+
 // Raw buffer over [base, base + bytes); base and bytes are the same in every
 // lane (the item's record), taken from the first so the resource is scalar.
 // readfirstlane returns int: each half goes through uint32_t before widening,
@@ -601,6 +606,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
         advance();
         have = false;
         __syncthreads();
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);   // the queue's reads, on every lane
         return;
     }
     // two copies of the split: after fresh loads it waits for all of them,
@@ -675,6 +681,12 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
         const float v = acc0[i] + (acc1[i] + acc2[i]);
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), yr, o * int(sizeof(float)), 0, 0);
     }
+    // With at most 16 memory operations outstanding (the stores above),
+    // every load issued before them has returned: the prefetched stage and
+    // the queue's reads.  Said here, on every lane, the compiler knows those
+    // registers are written when the loop comes round, and does not wait for
+    // the stores to copy them.
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt16);
 }
 
 // BF16x3 staging fits the FFT data region (the twiddle table stays); BF16x6
@@ -806,11 +818,17 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
                     default: fft_item<4096>(a, it, q, buf, lc); break;
                 }
             }
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);   // the queue's reads, on every lane (as above)
         }
         // Every item kind passes a barrier after all threads have read the
         // ring at the top, and no thread reads s_rec[r2] or writes s_item[r0]
         // before that, so the writes below need no barrier before them.
-        if (threadIdx.x < 16) reinterpret_cast<int32_t*>(&s_rec[r2])[threadIdx.x] = pre;
+        // The slot address is formed here from an opaque copy of the thread
+        // index: hoisted out of the loop it would hold a register the whole
+        // loop, one more than the matrix-core items leave free.
+        int lane16 = threadIdx.x;
+        asm volatile("" : "+v"(lane16));
+        if (lane16 < 16) reinterpret_cast<int32_t*>(&s_rec[r2])[lane16] = pre;
         if (threadIdx.x == 0) s_item[r0] = resolve(next3);
         __syncthreads();
         r0 = r1;
