@@ -499,6 +499,9 @@ __device__ __forceinline__ const float* tap_ptr(const BatchArgs& a, const BatchI
     return a.k + p.k_off + (a.rev ? p.k - 1 - t : t);
 }
 
+// Two consecutive samples, at any dword-aligned address.
+typedef float SamplePair __attribute__((ext_vector_type(2), aligned(4)));
+
 // A matrix-core item's window samples and first round of taps, in registers:
 // thread t holds window pairs e = 2 (t + u T) and the taps of tap-plane pair
 // e = 2 t.  Loaded either at the item's start or during the item before it.
@@ -523,13 +526,17 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
     c.pds = p.pds;
     const int t2 = 2 * tid();
     if (o0 >= c.p0 && o0 + W <= int64_t(c.p0) + c.n) {
-        const float* src = a.x + c.x_off + (o0 - c.p0);   // all interior: no border logic
+        // All interior: no border logic, and each pair is one 8-byte load
+        // (dword alignment is all a multi-dword global load needs).  Lanes
+        // past the window read the first two constant cells, which the stage
+        // store never uses.
+        const float* src = a.x + c.x_off + (o0 - c.p0);
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
             const int e = t2 + 2 * u * T;   // W is even: e + 1 < W whenever e < W
-            const float* pe = e < W ? src + e : a.consts;
-            st.w0[u] = pe[0];
-            st.w1[u] = pe[e < W ? 1 : 0];
+            const SamplePair v = *reinterpret_cast<const SamplePair*>(e < W ? src + e : a.consts);
+            st.w0[u] = v.x;
+            st.w1[u] = v.y;
         }
     } else {
 #pragma unroll
