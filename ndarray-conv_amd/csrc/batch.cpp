@@ -354,6 +354,7 @@ bool time_schedule(LaunchContext& ctx, const Schedule& S, const float* x, const 
     D.args.y = y;
     D.args.s = s;
     D.args.d = d;
+    D.args.dmul = batch_dmul(d);
     D.args.rev = 1;
     if (!D.set_borders(kS1Circular, kS1Circular, 0.f, 0.f)) return false;
     hipStream_t st = ctx.stream();
@@ -592,6 +593,7 @@ ndc_status batch_create(ndc_handle h, const ndc_batch_desc* d, Batch** out) {
         BatchArgs& a = b->D.args;
         a.s = b->s;
         a.d = b->d;
+        a.dmul = batch_dmul(b->d);
         a.rev = d->reverse ? 1 : 0;
         if (!b->D.set_borders(side1d(front.tag), side1d(back.tag), float(front.value.re), float(back.value.re)))
             return fail(h, NDC_ERR_OUT_OF_MEMORY, "batch metadata upload failed");

@@ -54,6 +54,11 @@ constexpr int kBf16x3MinTaps = 32;
 constexpr int kBatchQueues = 8;
 constexpr int kBatchQueueStride = 64;
 
+// ceil(2^31 / d), d >= 1: for 0 <= q < 2^20 and d <= 2^11, q / d is the high
+// word of 2 q * batch_dmul(d) (the error term q (2^31 mod d) / (d 2^31) stays
+// below 1 / d).
+constexpr uint32_t batch_dmul(int d) { return uint32_t(((uint64_t(1) << 31) + uint64_t(d) - 1) / uint64_t(d)); }
+
 constexpr int batch_lc(int cls) { return 256 << cls; }
 constexpr int batch_lines(int cls) { return kBatchPoints / batch_lc(cls); }
 // Quarter-circle twiddle tables of all classes, concatenated: class c has
@@ -115,6 +120,7 @@ struct BatchArgs {
     const float* consts;           // {0, cfront, cback, 0}: the cells constant pad positions read
     int32_t nitems, nkitems;
     int32_t s, d, rev;
+    uint32_t dmul;                 // batch_dmul(d)
     int32_t front, back;           // side tags of border1d.hpp
     float cfront, cback;           // constants of Const sides
 };

@@ -51,7 +51,7 @@ constexpr int kWaitVmcnt16 = 0x4F70;
 // lane (the item's record), taken from the first so the resource is scalar.
 // readfirstlane returns int: each half goes through uint32_t before widening,
 // or a low half with bit 31 set would sign-extend over the high half.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(float* base, int bytes) {
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(const float* base, int bytes) {
     const uint64_t b = reinterpret_cast<uint64_t>(base);
     const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))));
     const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))));
@@ -491,16 +491,28 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
 // P = 2 (BF16x3) issues t0*u0, t0*u1, t1*u0 per block; P = 3 (BF16x6) also
 // t1*u1, t2*u0, t0*u2.  Three accumulators keep the chains independent and
 // the small terms apart from the leading one.
-// Address of tap q of the dilated correlation kernel kc_dil (zero cell
-// outside [0, kdil) and between dilated taps).
-__device__ __forceinline__ const float* tap_ptr(const BatchArgs& a, const BatchItem& p, int kdil, int q) {
-    if (q < 0 || q >= kdil || q % a.d) return a.consts;
-    const int t = q / a.d;
-    return a.k + p.k_off + (a.rev ? p.k - 1 - t : t);
+// Tap q of the dilated correlation kernel kc_dil (zero outside [0, kdil) and
+// between dilated taps) through a raw buffer kr over the pair's k taps: an
+// index that is no tap is clamped to k, the first offset past the buffer.
+// q / d is mulhi(2 q, dmul), dmul = ceil(2^31 / d): exact for q < 2^20 when
+// d <= 2^11, and matrix-core items have (k - 1) d < 1024 (q = 0 is exact for
+// any d).
+__device__ __forceinline__ float dil_tap(const BatchArgs& a, __amdgpu_buffer_rsrc_t kr, int k, int kdil, int q) {
+    const uint32_t uq = uint32_t(q);
+    const uint32_t t = __umulhi(uq << 1, a.dmul);
+    const bool tap = uq < uint32_t(kdil) && t * uint32_t(a.d) == uq;
+    const uint32_t i = tap ? (a.rev ? uint32_t(k - 1) - t : t) : uint32_t(k);
+    return __builtin_bit_cast(float, uint32_t(__builtin_amdgcn_raw_buffer_load_b32(kr, int(i * 4), 0, 0)));
 }
 
-// Two consecutive samples, at any dword-aligned address.
-typedef float SamplePair __attribute__((ext_vector_type(2), aligned(4)));
+// Taps q0, q0 + 1, q0 + 2 of the dilated correlation kernel.
+__device__ __forceinline__ void taps3(const BatchArgs& a, const BatchItem& p, int kdil, int q0, float& f0, float& f1,
+                                      float& f2) {
+    const __amdgpu_buffer_rsrc_t kr = raw_buffer(a.k + p.k_off, p.k * int(sizeof(float)));
+    f0 = dil_tap(a, kr, p.k, kdil, q0);
+    f1 = dil_tap(a, kr, p.k, kdil, q0 + 1);
+    f2 = dil_tap(a, kr, p.k, kdil, q0 + 2);
+}
 
 // A matrix-core item's window samples and first round of taps, in registers:
 // thread t holds window pairs e = 2 (t + u T) and the taps of tap-plane pair
@@ -526,17 +538,20 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
     c.pds = p.pds;
     const int t2 = 2 * tid();
     if (o0 >= c.p0 && o0 + W <= int64_t(c.p0) + c.n) {
-        // All interior: no border logic, and each pair is one 8-byte load
-        // (dword alignment is all a multi-dword global load needs).  Lanes
-        // past the window read the first two constant cells, which the stage
-        // store never uses.
-        const float* src = a.x + c.x_off + (o0 - c.p0);
+        // All interior: a raw buffer over the window's W samples, each pair
+        // one 8-byte load at the lane's offset plus a scalar one (dword
+        // alignment is all a multi-dword load needs).  Lanes past the window
+        // read zero, which the stage store never uses.
+        const __amdgpu_buffer_rsrc_t wr = raw_buffer(a.x + c.x_off + (o0 - c.p0), W * int(sizeof(float)));
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
-            const int e = t2 + 2 * u * T;   // W is even: e + 1 < W whenever e < W
-            const SamplePair v = *reinterpret_cast<const SamplePair*>(e < W ? src + e : a.consts);
-            st.w0[u] = v.x;
-            st.w1[u] = v.y;
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(wr, t2 * int(sizeof(float)),
+                                                                u * 2 * T * int(sizeof(float)), 0);
+            // copied out before the bit cast: clang's bit cast of a vector
+            // element reads element 0 whatever the index
+            const uint32_t v0 = v[0], v1 = v[1];
+            st.w0[u] = __builtin_bit_cast(float, v0);
+            st.w1[u] = __builtin_bit_cast(float, v1);
         }
     } else {
 #pragma unroll
@@ -546,9 +561,7 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
             st.w1[u] = *(e < W ? pad_ptr(a, c, o0 + e + 1) : a.consts);
         }
     }
-    st.f0 = *tap_ptr(a, p, kdil, t2 - 32);
-    st.f1 = *tap_ptr(a, p, kdil, t2 - 31);
-    st.f2 = *tap_ptr(a, p, kdil, t2 - 30);
+    taps3(a, p, kdil, t2 - 32, st.f0, st.f1, st.f2);
 }
 
 // Splits item p's stage into the P bf16 planes in LDS: window plane t at
@@ -567,14 +580,18 @@ __device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const Batch
     const int W = kBatchMfmaTile + 16 * nkb + 32;
     const int ntp = 16 * nkb + 48;
     const int t2 = 2 * tid();
+    // Pair u sits at e = t2 + 2 u T, padded position padh(t2) + u (2 T) 5/4;
+    // pairs u < 8 are inside every window (W > kBatchMfmaTile).
+    static_assert(2 * T % 32 == 0 && 8 * 2 * T <= kBatchMfmaTile, "pair positions of the stage store");
+    const int h0 = padh(t2);
 #pragma unroll
     for (int u = 0; u < MfmaStage::U; ++u) {
-        const int e = t2 + 2 * u * T;
-        if (e < W) {
+        if (u < 8 || t2 + 2 * u * T < W) {
             uint32_t w[P];
             split_pair<P>(st.w0[u], st.w1[u], w);
 #pragma unroll
-            for (int q = 0; q < P; ++q) reinterpret_cast<uint32_t*>(wp + q * L::win_slots + padh(e))[0] = w[q];
+            for (int q = 0; q < P; ++q)
+                reinterpret_cast<uint32_t*>(wp + q * L::win_slots + h0 + u * (2 * T) * 5 / 4)[0] = w[q];
         }
     }
     // pairs of both tap copies from three consecutive taps
@@ -589,8 +606,11 @@ __device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const Batch
         }
     };
     if (t2 < ntp) put_taps(t2, st.f0, st.f1, st.f2);
-    for (int e = t2 + 2 * T; e < ntp; e += 2 * T)
-        put_taps(e, *tap_ptr(a, p, kdil, e - 32), *tap_ptr(a, p, kdil, e - 31), *tap_ptr(a, p, kdil, e - 30));
+    for (int e = t2 + 2 * T; e < ntp; e += 2 * T) {
+        float f0, f1, f2;
+        taps3(a, p, kdil, e - 32, f0, f1, f2);
+        put_taps(e, f0, f1, f2);
+    }
 }
 
 // Matrix-core tile: outputs [o0, o0 + cnt) of one pair with stride 1.  With
@@ -652,8 +672,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
                                                        acc, 0, 0, 0);
     };
     floatx16 acc0, acc1, acc2;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = acc2[i] = 0.f;
+    const floatx16 zero = {};
     auto products = [&](const u32x4* av, const u32x4* bv) {
         acc0 = mfma(av[0], bv[0], acc0);
         acc1 = mfma(av[0], bv[1], acc1);
@@ -668,6 +687,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     // the current block's products issue, with no register copies between
     u32x4 a0[P], b0[P], a1[P], b1[P];
     load(0, a0, b0);
+    acc0 = acc1 = acc2 = zero;
     int kb = 0;
     for (; kb + 1 < nkb; kb += 2) {
         load(kb + 1, a1, b1);

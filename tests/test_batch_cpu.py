@@ -59,3 +59,16 @@ def test_batch_plan_needs_a_device():
     # this container has no GPU: planning a batch fails cleanly
     with pytest.raises(nc.Error):
         nc.BatchPlan([10, 20], [3, 4], nc.ConvMode.Same, nc.PaddingMode.Circular)
+
+
+def test_dilation_multiplier_divides_exactly():
+    """Matrix-core items find tap q / d of a dilated kernel as the high word of
+    2 q * ceil(2^31 / d) (batch.hpp, batch_dmul): exhaustively equal to q // d
+    for every dilation up to 2^11 and every tap position an item reads
+    (q < 2^13)."""
+    q = np.arange(1 << 13, dtype=np.uint64)
+    for d in range(1, (1 << 11) + 1):
+        m = ((1 << 31) + d - 1) // d
+        assert m < (1 << 32)
+        t = ((2 * q) * np.uint64(m)) >> np.uint64(32)
+        np.testing.assert_array_equal(t, q // np.uint64(d), err_msg=f"d={d}")

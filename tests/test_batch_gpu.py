@@ -108,6 +108,21 @@ def test_batch_long_signals_and_kernels(gpu):
     print(check(xs, ks, "same", "circular", algos=["auto", "fft", "direct", "bf16x3", "bf16x6"]))
 
 
+@pytest.mark.parametrize("dil", [2, 3, 5])
+def test_batch_dilated_matrix_core(gpu, dil):
+    # dilated kernels up to the longest dilated length of a matrix-core item
+    # (1024), so their taps are staged in up to three rounds, both orientations
+    rng = np.random.default_rng(40 + dil)
+    kmax = (1024 - 1) // dil + 1
+    k = np.concatenate([[kmax, kmax - 1, 2, 1], rng.integers(1, kmax + 1, 8)])
+    n = rng.integers(4200, 20000, len(k))
+    xs = [rng.uniform(-1, 1, int(a)).astype(np.float32) for a in n]
+    ks = [rng.uniform(-1, 1, int(b)).astype(np.float32) for b in k]
+    for rev in (True, False):
+        for mode, pm in (("same", "circular"), ("full", "zeros")):
+            print(dil, rev, mode, check(xs, ks, mode, pm, dil, rev, algos=["bf16x3", "bf16x6", "auto_bf16x3"]))
+
+
 def test_batch_errors_follow_the_reference(gpu):
     x = [np.ones(10, np.float32), np.ones(0, np.float32)]
     k = [np.ones(3, np.float32), np.ones(3, np.float32)]
