@@ -653,18 +653,24 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     const int half = lane >> 5, m = lane & 31;
     const int wb = wave * 1024;
     // operands of K-block kb: A lane holds row m (a 32-output segment), B lane
-    // column m (an output offset); both K entries 16 kb + 8 half + i
+    // column m (an output offset); both K entries 16 kb + 8 half + i.  The A
+    // operand is window entry padh(wb + 32 m + 8 half + 16 kb), which is
+    // pa + 40 (kb / 2) + 16 (kb % 2) (8 half + 16 (kb % 2) < 32: a pair of
+    // blocks spans one padding step); the B operand is taps
+    // 16 kb + 8 half - m + i, i < 8, from the copy where they start even, at
+    // pb + 16 kb.  Both advance by constants, with no index arithmetic.
     const int par = (32 + 8 * half - m) & 1;   // parity of this lane's first tap index
-    auto load = [&](int kb, u32x4* av, u32x4* bv) {
-        const int ea = padh(wb + 32 * m + 16 * kb + 8 * half);
-        // taps 16 kb + 8 half - m + i, i < 8, from the copy where they start even
-        const int q = 32 + 16 * kb + 8 * half - m - par;
+    const uint16_t* pa = wp + padh(wb + 32 * m + 8 * half);
+    const uint16_t* pb = tp + par * L::taps + (32 + 8 * half - m - par);
+    auto load = [&](int j, int odd, u32x4* av, u32x4* bv) {   // block kb = 2 j + odd
+        const uint16_t* qa = pa + 40 * j + 16 * odd;
+        const uint16_t* qb = pb + 32 * j + 16 * odd;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            av[s] = *reinterpret_cast<const u32x4*>(wp + s * L::win_slots + ea);
-            const uint32_t* pb = reinterpret_cast<const uint32_t*>(tp + (2 * s + par) * L::taps + q);
+            av[s] = *reinterpret_cast<const u32x4*>(qa + s * L::win_slots);
+            const uint32_t* b = reinterpret_cast<const uint32_t*>(qb + 2 * s * L::taps);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) bv[s][i] = pb[i];
+            for (int i = 0; i < 4; ++i) bv[s][i] = b[i];
         }
     };
     auto mfma = [](const u32x4& x, const u32x4& y, floatx16 acc) {
@@ -672,7 +678,19 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
                                                        acc, 0, 0, 0);
     };
     floatx16 acc0, acc1, acc2;
+    // block 0's products start from the instruction's inline zero (no
+    // register clears)
     const floatx16 zero = {};
+    auto first = [&](const u32x4* av, const u32x4* bv) {
+        acc0 = mfma(av[0], bv[0], zero);
+        acc1 = mfma(av[0], bv[1], zero);
+        acc2 = mfma(av[1], bv[0], zero);
+        if constexpr (P == 3) {
+            acc1 = mfma(av[1], bv[1], acc1);
+            acc2 = mfma(av[2], bv[0], acc2);
+            acc1 = mfma(av[0], bv[2], acc1);
+        }
+    };
     auto products = [&](const u32x4* av, const u32x4* bv) {
         acc0 = mfma(av[0], bv[0], acc0);
         acc1 = mfma(av[0], bv[1], acc1);
@@ -686,16 +704,18 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     // two operand sets in turn: the next block's reads are in flight while
     // the current block's products issue, with no register copies between
     u32x4 a0[P], b0[P], a1[P], b1[P];
-    load(0, a0, b0);
-    acc0 = acc1 = acc2 = zero;
-    int kb = 0;
+    // block 0, then odd blocks in the second set and even ones in the first
+    load(0, 0, a0, b0);
+    first(a0, b0);
+    load(0, 1, a1, b1);   // nkb >= 2: kdil + 31 >= 32
+    int kb = 1;
     for (; kb + 1 < nkb; kb += 2) {
-        load(kb + 1, a1, b1);
-        products(a0, b0);
-        if (kb + 2 < nkb) load(kb + 2, a0, b0);
+        load((kb + 1) >> 1, 0, a0, b0);
         products(a1, b1);
+        if (kb + 2 < nkb) load((kb + 2) >> 1, 1, a1, b1);
+        products(a0, b0);
     }
-    if (kb < nkb) products(a0, b0);
+    if (kb < nkb) products(a1, b1);
     // D lane holds D[8 (i/4) + 4 half + i%4][m].  Every wave issues all 16
     // stores: the tile's outputs are a raw buffer of cnt floats, and the
     // hardware drops the stores of lanes past its end.  With no store
