@@ -64,9 +64,17 @@ template <int P>
 struct MfmaLds {
     static constexpr int nkb = (kBatchMfmaMaxKdil + 31 + 15) / 16;
     static constexpr int win = kBatchMfmaTile + 16 * nkb + 32;         // window samples
-    static constexpr int win_slots = win + 8 * ((win + 31) / 32);      // padded plane length
-    static constexpr int taps = 16 * nkb + 48;                         // tap plane length (even)
-    static constexpr size_t bytes = size_t(P * win_slots + 2 * P * taps) * 2;
+    static constexpr int win_slots = win + 8 * ((win - 1) / 32);       // padded plane length: padh(win - 1) + 1
+    static constexpr int taps = 16 * nkb + 48;                         // tap copy length (even)
+    // Copy 1 of a term starts copy1 entries after copy 0, 17 banks further
+    // on: in one B read the even lanes take 16 consecutive dwords of copy 0
+    // ending at some bank b and the odd lanes 16 of copy 1 ending at b + 16
+    // (the shift less one), so the two sets share no bank.  A term's two
+    // copies take term entries.
+    static constexpr int copy1 = taps + 18;
+    static constexpr int term = taps + copy1;
+    static_assert((copy1 / 2) % 32 == 17, "tap copies 17 banks apart");
+    static constexpr size_t bytes = size_t(P * win_slots + P * term) * 2;
 };
 
 // dynamic LDS of both kernels: the larger of the FFT layout and BF16x6 staging
@@ -566,7 +574,7 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
 
 // Splits item p's stage into the P bf16 planes in LDS: window plane t at
 // wp + t * win_slots (pairs stored as one 32-bit word; padh pads 16 bytes per
-// 64), tap plane t, copy c at tp + (2 t + c) * taps, copy 0 holding tap q at
+// 64), tap term t, copy c at tp + t * term + c * copy1, copy 0 holding tap q at
 // 32 + q and copy 1 at 31 + q, zero outside [0, kdil).  Tap rounds past the
 // first are loaded here.  No barrier.
 template <int P>
@@ -601,8 +609,8 @@ __device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const Batch
         split_pair<P>(t1, t2_, c1);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            reinterpret_cast<uint32_t*>(tp + (2 * q) * L::taps + e)[0] = c0[q];
-            reinterpret_cast<uint32_t*>(tp + (2 * q + 1) * L::taps + e)[0] = c1[q];
+            reinterpret_cast<uint32_t*>(tp + q * L::term + e)[0] = c0[q];
+            reinterpret_cast<uint32_t*>(tp + q * L::term + L::copy1 + e)[0] = c1[q];
         }
     };
     if (t2 < ntp) put_taps(t2, st.f0, st.f1, st.f2);
@@ -661,14 +669,14 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     // pb + 16 kb.  Both advance by constants, with no index arithmetic.
     const int par = (32 + 8 * half - m) & 1;   // parity of this lane's first tap index
     const uint16_t* pa = wp + padh(wb + 32 * m + 8 * half);
-    const uint16_t* pb = tp + par * L::taps + (32 + 8 * half - m - par);
+    const uint16_t* pb = tp + par * L::copy1 + (32 + 8 * half - m - par);
     auto load = [&](int j, int odd, u32x4* av, u32x4* bv) {   // block kb = 2 j + odd
         const uint16_t* qa = pa + 40 * j + 16 * odd;
         const uint16_t* qb = pb + 32 * j + 16 * odd;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             av[s] = *reinterpret_cast<const u32x4*>(qa + s * L::win_slots);
-            const uint32_t* b = reinterpret_cast<const uint32_t*>(qb + 2 * s * L::taps);
+            const uint32_t* b = reinterpret_cast<const uint32_t*>(qb + s * L::term);
 #pragma unroll
             for (int i = 0; i < 4; ++i) bv[s][i] = b[i];
         }
