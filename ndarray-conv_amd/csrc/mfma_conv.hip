@@ -43,6 +43,13 @@ constexpr int kMfmaX3MinTaps = 32;   // fewest kernel taps BF16x3 is used for
 // rotation aligned with the batches
 template <int Q> constexpr int rows_per_batch() { return Q == 3 ? 15 : 16; }
 
+// Entries of one staged BF16 plane (stride-1 columns): sr * w rounded up to
+// 16 more than a multiple of 32.  A K-block read takes its even lanes from
+// plane 0 and its odd lanes from the plane shifted by one element, `plane`
+// 32-bit words further on; each set is 16 consecutive words, so with the
+// planes 16 banks apart the two sets never share a bank.
+constexpr int bf16_plane(int sr, int w) { return (sr * w + 15) / 32 * 32 + 16; }
+
 struct MfmaArgs {
     const float* x;
     const float* k;
@@ -203,7 +210,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_conv_kernel(MfmaArgs a) {
 #pragma unroll
                 for (int i = 0; i < kPer; ++i) raw[i] = a.x[off[i]];   // offset 0 (a valid element) where not used
             };
-            const int plane = kSR * a.W;   // entries of one BF16 plane (D1)
+            const int plane = D1 ? bf16_plane(kSR, a.W) : kSR * a.W;   // entries of one BF16 plane (D1)
             auto store = [&](uint32_t* dst) {
                 uint16_t* h16 = reinterpret_cast<uint16_t*>(dst);
                 bool bad = false;
@@ -506,9 +513,10 @@ ndc_status launch_direct_mfma(const Plan& p, const void* x, const void* k, void*
     a.BY = int32_t(by);
     const int64_t nty = (a.OY + a.BY - 1) / a.BY;
     const int64_t blocks = nty * a.ntx;
-    // two buffers of SR x W staged entries: four 2-byte BF16 planes per entry
-    // (D1) or one 32-bit (hi, lo) word
-    const size_t smem = size_t(2) * SR * a.W * (d1 ? 4 * sizeof(uint16_t) : sizeof(uint32_t));
+    // two buffers of SR x W staged entries: four 2-byte BF16 planes of
+    // bf16_plane(SR, W) entries (D1) or one 32-bit (hi, lo) word per entry
+    const size_t smem = d1 ? size_t(2) * 4 * bf16_plane(SR, a.W) * sizeof(uint16_t)
+                           : size_t(2) * SR * a.W * sizeof(uint32_t);
     void (*fn)(const MfmaArgs&, unsigned, size_t, LaunchContext&) = nullptr;
     const void* kp = x3 ? pick<true>(nkb, Q, d1, &fn) : pick<false>(nkb, Q, d1, &fn);
     if (!kp || !fn) return NDC_ERR_UNSUPPORTED;
