@@ -532,6 +532,17 @@ struct MfmaStage {
 };
 static_assert(MfmaLds<2>::win == MfmaLds<3>::win, "both splits read the same window");
 
+// Window pair of thread t in each round of 2 T samples: the 32 lanes of a
+// half-wave take two runs of 16 pairs, 64 pairs apart.  padh adds 4 words
+// per 16, so the runs' words are 80 apart (16 banks) and a half-wave's
+// stage store touches every bank once; 32 consecutive pairs would span 36
+// words and write 4 banks twice.
+__device__ __forceinline__ int stage_pair(int t) {
+    const int h = t >> 5, l = t & 31;
+    return 16 * ((h & 3) + 8 * (h >> 2) + 4 * (l >> 4)) + (l & 15);
+}
+static_assert(T == 256, "stage_pair maps 256 threads onto 16 runs of 16 pairs");
+
 // Issues the loads of item p's stage (no waits).  The window covers padded
 // positions o0 + e for every sample an A operand reads.
 __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchItem& p, MfmaStage& st) {
@@ -545,6 +556,7 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
     c.p0 = p.p0;
     c.pds = p.pds;
     const int t2 = 2 * tid();
+    const int e2 = 2 * stage_pair(tid());   // the thread's first window pair
     if (o0 >= c.p0 && o0 + W <= int64_t(c.p0) + c.n) {
         // All interior: a raw buffer over the window's W samples, each pair
         // one 8-byte load at the lane's offset plus a scalar one (dword
@@ -553,7 +565,7 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
         const __amdgpu_buffer_rsrc_t wr = raw_buffer(a.x + c.x_off + (o0 - c.p0), W * int(sizeof(float)));
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b64(wr, t2 * int(sizeof(float)),
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(wr, e2 * int(sizeof(float)),
                                                                 u * 2 * T * int(sizeof(float)), 0);
             // copied out before the bit cast: clang's bit cast of a vector
             // element reads element 0 whatever the index
@@ -564,7 +576,7 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
     } else {
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
-            const int e = t2 + 2 * u * T;
+            const int e = e2 + 2 * u * T;
             st.w0[u] = *(e < W ? pad_ptr(a, c, o0 + e) : a.consts);
             st.w1[u] = *(e < W ? pad_ptr(a, c, o0 + e + 1) : a.consts);
         }
@@ -588,13 +600,14 @@ __device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const Batch
     const int W = kBatchMfmaTile + 16 * nkb + 32;
     const int ntp = 16 * nkb + 48;
     const int t2 = 2 * tid();
-    // Pair u sits at e = t2 + 2 u T, padded position padh(t2) + u (2 T) 5/4;
+    const int e2 = 2 * stage_pair(tid());
+    // Pair u sits at e = e2 + 2 u T, padded position padh(e2) + u (2 T) 5/4;
     // pairs u < 8 are inside every window (W > kBatchMfmaTile).
     static_assert(2 * T % 32 == 0 && 8 * 2 * T <= kBatchMfmaTile, "pair positions of the stage store");
-    const int h0 = padh(t2);
+    const int h0 = padh(e2);
 #pragma unroll
     for (int u = 0; u < MfmaStage::U; ++u) {
-        if (u < 8 || t2 + 2 * u * T < W) {
+        if (u < 8 || e2 + 2 * u * T < W) {
             uint32_t w[P];
             split_pair<P>(st.w0[u], st.w1[u], w);
 #pragma unroll
