@@ -16,6 +16,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "batch.hpp"
@@ -497,8 +498,7 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
 // its own, so an A operand is one aligned 16-byte read per plane; the taps are
 // kept in two copies shifted by one element so every B read is 4-byte aligned.
 // P = 2 (BF16x3) issues t0*u0, t0*u1, t1*u0 per block; P = 3 (BF16x6) also
-// t1*u1, t2*u0, t0*u2.  Three accumulators keep the chains independent and
-// the small terms apart from the leading one.
+// t1*u1, t2*u0, t0*u2.
 // Tap q of the dilated correlation kernel kc_dil (zero outside [0, kdil) and
 // between dilated taps) through a raw buffer kr over the pair's k taps: an
 // index that is no tap is clamped to k, the first offset past the buffer.
@@ -698,30 +698,40 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, x), __builtin_bit_cast(bf16x8, y),
                                                        acc, 0, 0, 0);
     };
-    floatx16 acc0, acc1, acc2;
+    // Two accumulators, written strictly in turn: each matrix instruction
+    // has one independent instruction between it and the one it depends on.
+    // P = 2 takes three products per block, so the block's parity picks the
+    // accumulator it starts with; P = 3 takes six.  Every product lands in
+    // one of them and the output is their sum (FP32 accumulation, as an FP32
+    // dot product; the split's own error is 2^-16 per product).
+    floatx16 acc0, acc1;
     // block 0's products start from the instruction's inline zero (no
     // register clears)
     const floatx16 zero = {};
     auto first = [&](const u32x4* av, const u32x4* bv) {
         acc0 = mfma(av[0], bv[0], zero);
         acc1 = mfma(av[0], bv[1], zero);
-        acc2 = mfma(av[1], bv[0], zero);
+        acc0 = mfma(av[1], bv[0], acc0);
         if constexpr (P == 3) {
             acc1 = mfma(av[1], bv[1], acc1);
-            acc2 = mfma(av[2], bv[0], acc2);
+            acc0 = mfma(av[2], bv[0], acc0);
             acc1 = mfma(av[0], bv[2], acc1);
         }
     };
-    auto products = [&](const u32x4* av, const u32x4* bv) {
-        acc0 = mfma(av[0], bv[0], acc0);
-        acc1 = mfma(av[0], bv[1], acc1);
-        acc2 = mfma(av[1], bv[0], acc2);
+    auto products = [&](const u32x4* av, const u32x4* bv, auto odd) {
+        floatx16& x = (P == 2 && odd.value) ? acc1 : acc0;
+        floatx16& y = (P == 2 && odd.value) ? acc0 : acc1;
+        x = mfma(av[0], bv[0], x);
+        y = mfma(av[0], bv[1], y);
+        x = mfma(av[1], bv[0], x);
         if constexpr (P == 3) {
-            acc1 = mfma(av[1], bv[1], acc1);
-            acc2 = mfma(av[2], bv[0], acc2);
-            acc1 = mfma(av[0], bv[2], acc1);
+            y = mfma(av[1], bv[1], y);
+            x = mfma(av[2], bv[0], x);
+            y = mfma(av[0], bv[2], y);
         }
     };
+    using Even = std::integral_constant<bool, false>;
+    using Odd = std::integral_constant<bool, true>;
     // two operand sets in turn: the next block's reads are in flight while
     // the current block's products issue, with no register copies between
     u32x4 a0[P], b0[P], a1[P], b1[P];
@@ -732,11 +742,11 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     int kb = 1;
     for (; kb + 1 < nkb; kb += 2) {
         load((kb + 1) >> 1, 0, a0, b0);
-        products(a1, b1);
+        products(a1, b1, Odd{});
         if (kb + 2 < nkb) load((kb + 2) >> 1, 1, a1, b1);
-        products(a0, b0);
+        products(a0, b0, Even{});
     }
-    if (kb < nkb) products(a1, b1);
+    if (kb < nkb) products(a1, b1, Odd{});
     // D lane holds D[8 (i/4) + 4 half + i%4][m].  Every wave issues all 16
     // stores: the tile's outputs are a raw buffer of cnt floats, and the
     // hardware drops the stores of lanes past its end.  With no store
@@ -746,7 +756,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int o = wb + 32 * (8 * (i / 4) + 4 * half + (i % 4)) + m;
-        const float v = acc0[i] + (acc1[i] + acc2[i]);
+        const float v = acc0[i] + acc1[i];
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), yr, o * int(sizeof(float)), 0, 0);
     }
     // With at most 16 memory operations outstanding (the stores above),
