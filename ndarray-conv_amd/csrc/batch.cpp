@@ -164,6 +164,7 @@ struct Schedule {
     std::vector<int32_t> kpairs;
     std::vector<BatchKItem> kitems;
     int64_t ks_total = 0;
+    int32_t nfirst = 0;            // direct and FFT items, ahead of the matrix-core items
     double est_ms = 0;
     double kspec_bytes = 0, main_bytes = 0, main_flops = 0;
 };
@@ -272,11 +273,16 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
         for (int32_t f = 0; f < nk; f += G) S.kitems.push_back(BatchKItem{c, kb + f, std::min(G, nk - f), 0});
     }
     // longest items first: the queue hands them out in this order, so the
-    // tail of the launch is made of the shortest items
+    // tail of the launch is made of the shortest items.  The direct and FFT
+    // items form the first range, the matrix-core items the second.
     std::stable_sort(work.begin(), work.end(),
                      [](const std::pair<double, BatchItem>& a, const std::pair<double, BatchItem>& b) {
                          return a.first > b.first;
                      });
+    const auto first_end = std::stable_partition(work.begin(), work.end(), [](const std::pair<double, BatchItem>& w) {
+        return w.second.kind != kBatchKindMfma && w.second.kind != kBatchKindMfma6;
+    });
+    S.nfirst = int32_t(first_end - work.begin());
     S.items.reserve(work.size());
     for (auto& w : work) {
         S.items.push_back(w.second);
@@ -308,7 +314,7 @@ struct DevSchedule {
             !lines.upload(S.lines.data(), S.lines.size() * sizeof(BatchLine)) ||
             !kpairs.upload(S.kpairs.data(), S.kpairs.size() * sizeof(int32_t)) ||
             !kitems.upload(S.kitems.data(), S.kitems.size() * sizeof(BatchKItem)) ||
-            !kspec.ensure(size_t(S.ks_total) * sizeof(float2)) || !counter.ensure(size_t(kBatchQueues) * kBatchQueueStride * sizeof(int32_t)))
+            !kspec.ensure(size_t(S.ks_total) * sizeof(float2)) || !counter.ensure(size_t(kBatchHeads) * kBatchQueueStride * sizeof(int32_t)))
             return false;
         args.pairs = static_cast<const BatchPair*>(pairs.p);
         args.items = static_cast<const BatchItem*>(items.p);
@@ -319,6 +325,7 @@ struct DevSchedule {
         args.counter = static_cast<int32_t*>(counter.p);
         args.tw = tw;
         args.nitems = int32_t(S.items.size());
+        args.nfirst = S.nfirst;
         args.nkitems = int32_t(S.kitems.size());
         grid = int(std::min<int64_t>(int64_t(S.items.size()), int64_t(blocks)));
         return true;

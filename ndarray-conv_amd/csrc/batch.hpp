@@ -47,12 +47,18 @@ constexpr int kBatchKindMfma6 = 2 + kBatchClasses;   // BF16x6 matrix-core tile 
 // within 5 * 2^-24.
 constexpr int kBf16x3MinTaps = 32;
 
-// The work queue is kBatchQueues sub-queues: sub-queue q holds items q,
-// q + kBatchQueues, q + 2 kBatchQueues, ... of the longest-first item list.
-// One head per sub-queue, kBatchQueueStride ints (256 bytes) apart: a single
-// head shared by every workgroup serializes the pops (about 12 ns each).
+// The work queue is two ranges of the item list, each longest-first: the
+// direct and FFT items [0, nfirst), then the matrix-core items [nfirst,
+// nitems).  Workgroups below nfirst drain the first range, then all drain
+// the second.  Each
+// range is kBatchQueues sub-queues: sub-queue q of a range starting at lo
+// holds items lo + q, lo + q + kBatchQueues, ...  One head per sub-queue,
+// kBatchQueueStride ints (256 bytes) apart, the second range's heads after
+// the first's: a single head shared by every workgroup serializes the pops
+// (about 12 ns each).
 constexpr int kBatchQueues = 8;
 constexpr int kBatchQueueStride = 64;
+constexpr int kBatchHeads = 2 * kBatchQueues;
 
 // ceil(2^31 / d), d >= 1: for 0 <= q < 2^20 and d <= 2^11, q / d is the high
 // word of 2 q * batch_dmul(d) (the error term q (2^31 mod d) / (d 2^31) stays
@@ -115,10 +121,11 @@ struct BatchArgs {
     const int32_t* kpairs;         // pairs of the kernel-spectrum pre-pass, grouped by class
     const BatchKItem* kitems;
     float2* kspec;
-    int32_t* counter;              // sub-queue heads (kBatchQueueStride apart), reset by the pre-pass
+    int32_t* counter;              // kBatchHeads sub-queue heads (kBatchQueueStride apart), reset by the pre-pass
     const float2* tw;              // W_L^r, r < L/4, of class c at batch_tw_offset(c) (L = 2 LC)
     const float* consts;           // {0, cfront, cback, 0}: the cells constant pad positions read
     int32_t nitems, nkitems;
+    int32_t nfirst;                // items of the first range (direct and FFT)
     int32_t s, d, rev;
     uint32_t dmul;                 // batch_dmul(d)
     int32_t front, back;           // side tags of border1d.hpp

@@ -779,7 +779,7 @@ __device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
 __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a) {
     extern __shared__ C smem[];
     __shared__ LineCtx lc[16];
-    if (blockIdx.x == 0 && threadIdx.x < kBatchQueues) a.counter[threadIdx.x * kBatchQueueStride] = 0;   // sub-queue heads
+    if (blockIdx.x == 0 && threadIdx.x < kBatchHeads) a.counter[threadIdx.x * kBatchQueueStride] = 0;   // sub-queue heads
     if (int(blockIdx.x) >= a.nkitems) return;
     const BatchKItem it = a.kitems[blockIdx.x];
     C* q = smem;
@@ -796,18 +796,77 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a)
     }
 }
 
-// Persistent: every workgroup takes items from the queue head until the
-// queue is empty (the only exit, reached by every workgroup).
-__global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs a) {
+// LDS of the persistent loop beside the dynamic allocation: the FFT lines'
+// context and the queue ring.
+__shared__ LineCtx g_lc[16];
+__shared__ int g_item[3];
+__shared__ BatchItem g_rec[3];
+
+// Persistent: the workgroup takes items of one range of the queue until the
+// range is empty (the only exit, reached by every workgroup): the direct and
+// FFT items (M false) or the matrix-core items (M true).  Each range's loop
+// is a function of its own, called once per workgroup, so the registers of
+// one item kind's code do not crowd the other's: with the FFT code in the
+// same loop the matrix-core items' scalar values spilled to vector lanes
+// (about a fifth of their vector instructions).
+typedef const __attribute__((address_space(4))) BatchArgs* KernargArgs;
+
+// The launch arguments, field by field from the kernel argument segment
+// (where the kernel's one parameter lies, at ap): scalar loads, whose
+// registers the loop never waits for.  A function argument arrives in vector
+// registers, so the pointer is first made uniform; loads through a vector
+// address, or a copy through a generic pointer, land in vector registers,
+// and the compiler, unable to tell those loads finished, waits for every
+// load of the wave at their first use inside the loop.  (The kernel argument
+// segment's own intrinsic is valid in kernels only: in a called function it
+// yields a null pointer.)
+__device__ __forceinline__ BatchArgs kernel_args(KernargArgs arg) {
+    const uint64_t u = reinterpret_cast<uint64_t>(arg);
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u))));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(u >> 32))));
+    const KernargArgs ap = reinterpret_cast<KernargArgs>((uint64_t(hi) << 32) | uint64_t(lo));
+    BatchArgs a;
+    a.x = ap->x;
+    a.k = ap->k;
+    a.y = ap->y;
+    a.pairs = ap->pairs;
+    a.items = ap->items;
+    a.lines = ap->lines;
+    a.kpairs = ap->kpairs;
+    a.kitems = ap->kitems;
+    a.kspec = ap->kspec;
+    a.counter = ap->counter;
+    a.tw = ap->tw;
+    a.consts = ap->consts;
+    a.nitems = ap->nitems;
+    a.nkitems = ap->nkitems;
+    a.nfirst = ap->nfirst;
+    a.s = ap->s;
+    a.d = ap->d;
+    a.rev = ap->rev;
+    a.dmul = ap->dmul;
+    a.front = ap->front;
+    a.back = ap->back;
+    a.cfront = ap->cfront;
+    a.cback = ap->cback;
+    return a;
+}
+
+template <bool M>
+__device__ __attribute__((noinline)) void drain(KernargArgs ap) {
+    const BatchArgs a = kernel_args(ap);
     extern __shared__ C smem[];
-    __shared__ LineCtx lc[16];
-    __shared__ int s_item[3];
-    __shared__ BatchItem s_rec[3];
+    LineCtx* lc = g_lc;
+    int* s_item = g_item;
+    BatchItem* s_rec = g_rec;
     C* q = smem;
     C* buf = smem + kTwSlots;
     float* win = reinterpret_cast<float*>(buf);
     float* taps = win + kWinSlots;
     int cur = -1;   // class whose twiddle table is in LDS
+    const int lo = M ? a.nfirst : 0, hi = M ? a.nitems : a.nfirst;
+    if (lo >= hi) return;   // uniform: an empty range costs no queue pops
+    int32_t* heads = a.counter + (M ? kBatchQueues * kBatchQueueStride : 0);
     // The queue runs three items ahead of the work.  While item i (ring slot
     // r0) is processed, the records of items i and i+1 are in s_rec[r0] and
     // s_rec[r1], item i+2's record (16 dwords, one per lane of the first 16
@@ -815,18 +874,18 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
     // later items go to slots no thread reads in the same iteration.
     //
     // Thread 0 draws tickets from sub-queue cq, starting with blockIdx.x mod
-    // kBatchQueues; ticket t of sub-queue q is item q + t kBatchQueues.  An
-    // empty sub-queue stays empty, so once cq is empty thread 0 moves on
-    // through the others in turn, and an index >= nitems after visiting all
-    // of them means every sub-queue is empty.
+    // kBatchQueues; ticket t of sub-queue q is item lo + q + t kBatchQueues.
+    // An empty sub-queue stays empty, so once cq is empty thread 0 moves on
+    // through the others in turn, and an index >= hi after visiting all of
+    // them means every sub-queue is empty.
     int cq = blockIdx.x % kBatchQueues, moved = 0;
-    auto ticket = [&]() { return atomicAdd(a.counter + cq * kBatchQueueStride, 1); };
+    auto ticket = [&]() { return atomicAdd(heads + cq * kBatchQueueStride, 1); };
     auto resolve = [&](int t) {
-        int g = cq + t * kBatchQueues;
-        while (g >= a.nitems && moved < kBatchQueues - 1) {
+        int g = lo + cq + t * kBatchQueues;
+        while (g >= hi && moved < kBatchQueues - 1) {
             cq = cq + 1 == kBatchQueues ? 0 : cq + 1;
             ++moved;
-            g = cq + ticket() * kBatchQueues;
+            g = lo + cq + ticket() * kBatchQueues;
         }
         return g;
     };
@@ -838,47 +897,42 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
     __syncthreads();
     {
         const int r = threadIdx.x >> 4, i = s_item[r & 1];
-        if (threadIdx.x < 32 && i < a.nitems)
+        if (threadIdx.x < 32 && i < hi)
             reinterpret_cast<int32_t*>(&s_rec[r])[threadIdx.x & 15] =
                 reinterpret_cast<const int32_t*>(a.items + i)[threadIdx.x & 15];
     }
     __syncthreads();
     MfmaStage st;
     bool have = false;   // st holds the current item's stage
+    // The queue's reads for items i+2 and i+3 (the ticket is resolved at the
+    // item's end), carried across iterations and written only by their
+    // loads: a register write over one that a load may still be filling
+    // makes the compiler wait for every load of the wave (the prefetched
+    // stage included).  Every lane loads pre (dword tid mod 16 of item i+2's
+    // record, or of item 0 past the end of the range); thread 0 draws next3,
+    // and the other lanes keep whatever the register held.
+    int32_t pre = 0;
+    int next3 = 0;
     for (int r0 = 0;;) {
         const int r1 = r0 == 2 ? 0 : r0 + 1, r2 = r1 == 2 ? 0 : r1 + 1;
         const int idx = s_item[r0];
-        if (idx >= a.nitems) break;   // uniform: every workgroup leaves once the queue is empty
-        // the queue's reads for items i+2 and i+3 (the ticket is resolved at
-        // the item's end).  Neither register is initialized: a register write
-        // over one that a load may still be filling makes the compiler wait
-        // for every load of the wave (the prefetched stage included).  Every
-        // lane loads pre (dword tid mod 16 of item i+2's record, or of item 0
-        // past the end of the queue), so pre is only ever written by loads,
-        // whose returns are ordered; thread 0 draws next3.
-        int32_t pre;
-        int next3;
+        if (idx >= hi) break;   // uniform: every workgroup leaves once the range is empty
         auto advance = [&]() {
             const int i2 = s_item[r2];
-            pre = reinterpret_cast<const int32_t*>(a.items + (i2 < a.nitems ? i2 : 0))[threadIdx.x & 15];
+            pre = reinterpret_cast<const int32_t*>(a.items + (i2 < hi ? i2 : 0))[threadIdx.x & 15];
             if (threadIdx.x == 0) next3 = ticket();
         };
         const BatchItem& it = s_rec[r0];
         const int kind = it.kind;
-        const bool mfma_next = s_item[r1] < a.nitems &&
-                               (s_rec[r1].kind == kBatchKindMfma || s_rec[r1].kind == kBatchKindMfma6);
-        const BatchItem* next = mfma_next ? &s_rec[r1] : nullptr;
-        if (kind == kBatchKindMfma) {
-            mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf), st, have, next, advance);
-        } else if (kind == kBatchKindMfma6) {
-            mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem), st, have, next, advance);
-            cur = -1;   // the staging overwrote the twiddle table
+        if constexpr (M) {
+            // every item of this range is a matrix-core item
+            const BatchItem* next = s_item[r1] < hi ? &s_rec[r1] : nullptr;
+            if (kind == kBatchKindMfma)
+                mfma_item<2>(a, it, reinterpret_cast<uint16_t*>(buf), st, have, next, advance);
+            else
+                mfma_item<3>(a, it, reinterpret_cast<uint16_t*>(smem), st, have, next, advance);
         } else {
             advance();
-            have = false;
-#pragma unroll
-            for (int u = 0; u < MfmaStage::U; ++u) st.w0[u] = st.w1[u] = 0.f;   // st is dead here
-            st.f0 = st.f1 = st.f2 = 0.f;
             if (kind == 0) {
                 direct_item(a, it, win, taps);
             } else {
@@ -911,6 +965,19 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         __syncthreads();
         r0 = r1;
     }
+}
+
+// The first min(grid, nfirst) workgroups drain the first range, then every
+// workgroup the second: a workgroup that finds a range empty pops each of
+// its sub-queues once more on the way out, so the first range's few items
+// are left to that many workgroups.
+__global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs a) {
+    const KernargArgs ap = (KernargArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    if (int(blockIdx.x) < a.nfirst) {
+        drain<false>(ap);
+        __syncthreads();   // the ring slots are reused by the second range
+    }
+    drain<true>(ap);
 }
 
 }  // namespace
