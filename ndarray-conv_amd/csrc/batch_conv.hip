@@ -804,11 +804,12 @@ __shared__ BatchItem g_rec[3];
 
 // Persistent: the workgroup takes items of one range of the queue until the
 // range is empty (the only exit, reached by every workgroup): the direct and
-// FFT items (M false) or the matrix-core items (M true).  Each range's loop
-// is a function of its own, called once per workgroup, so the registers of
-// one item kind's code do not crowd the other's: with the FFT code in the
-// same loop the matrix-core items' scalar values spilled to vector lanes
-// (about a fifth of their vector instructions).
+// FFT items (M false) or the matrix-core items (M true).  The matrix-core
+// range's loop is a function of its own, called once per workgroup, so the
+// FFT code's registers do not crowd it: in one loop with the FFT code the
+// matrix-core items' scalar values spilled to vector lanes (about a fifth of
+// their vector instructions).  The first range's loop stays in the kernel,
+// which has the larger scalar register budget (no call convention).
 typedef const __attribute__((address_space(4))) BatchArgs* KernargArgs;
 
 // The launch arguments, field by field from the kernel argument segment
@@ -853,7 +854,7 @@ __device__ __forceinline__ BatchArgs kernel_args(KernargArgs arg) {
 }
 
 template <bool M>
-__device__ __attribute__((noinline)) void drain(KernargArgs ap) {
+__device__ __forceinline__ void drain(KernargArgs ap) {
     const BatchArgs a = kernel_args(ap);
     extern __shared__ C smem[];
     LineCtx* lc = g_lc;
@@ -967,6 +968,9 @@ __device__ __attribute__((noinline)) void drain(KernargArgs ap) {
     }
 }
 
+// The matrix-core range's loop, called once per workgroup.
+__device__ __attribute__((noinline)) void drain_matrix_core(KernargArgs ap) { drain<true>(ap); }
+
 // The first min(grid, nfirst) workgroups drain the first range, then every
 // workgroup the second: a workgroup that finds a range empty pops each of
 // its sub-queues once more on the way out, so the first range's few items
@@ -977,7 +981,7 @@ __global__ void __launch_bounds__(kBatchThreads, 3) batch_main_kernel(BatchArgs 
         drain<false>(ap);
         __syncthreads();   // the ring slots are reused by the second range
     }
-    drain<true>(ap);
+    drain_matrix_core(ap);
 }
 
 }  // namespace
