@@ -523,25 +523,26 @@ __device__ __forceinline__ void taps3(const BatchArgs& a, const BatchItem& p, in
 }
 
 // A matrix-core item's window samples and first round of taps, in registers:
-// thread t holds window pairs e = 2 (t + u T) and the taps of tap-plane pair
-// e = 2 t.  Loaded either at the item's start or during the item before it.
+// thread t holds window quads e = 4 stage_quad(t) + u 4 T (four consecutive
+// samples each) and the taps of tap-plane pair e = 2 t.  Loaded either at
+// the item's start or during the item before it.
 struct MfmaStage {
-    static constexpr int U = (MfmaLds<3>::win + 2 * T - 1) / (2 * T);
-    float w0[U], w1[U];
+    static constexpr int U = (MfmaLds<3>::win + 4 * T - 1) / (4 * T);
+    float w[U][4];
     float f0, f1, f2;
 };
 static_assert(MfmaLds<2>::win == MfmaLds<3>::win, "both splits read the same window");
 
-// Window pair of thread t in each round of 2 T samples: the 32 lanes of a
-// half-wave take two runs of 16 pairs, 64 pairs apart.  padh adds 4 words
-// per 16, so the runs' words are 80 apart (16 banks) and a half-wave's
-// stage store touches every bank once; 32 consecutive pairs would span 36
-// words and write 4 banks twice.
-__device__ __forceinline__ int stage_pair(int t) {
-    const int h = t >> 5, l = t & 31;
-    return 16 * ((h & 3) + 8 * (h >> 2) + 4 * (l >> 4)) + (l & 15);
+// Window quad of thread t in each round of 4 T samples.  A quad's two pairs
+// are two consecutive words of a BF16 plane (padh keeps 4-aligned samples'
+// pairs together), stored as one 8-byte write.  padh adds 4 words per 16, so
+// a run of 8 quads fills 16 words and runs 4 apart are 80 words (16 banks)
+// apart: each 16-lane group takes two such runs and writes every bank once.
+__device__ __forceinline__ int stage_quad(int t) {
+    const int g = t >> 4, l = t & 15;
+    return 8 * ((g & 3) + 8 * (g >> 2) + 4 * (l >> 3)) + (l & 7);
 }
-static_assert(T == 256, "stage_pair maps 256 threads onto 16 runs of 16 pairs");
+static_assert(T == 256, "stage_quad maps 256 threads onto 32 runs of 8 quads");
 
 // Issues the loads of item p's stage (no waits).  The window covers padded
 // positions o0 + e for every sample an A operand reads.
@@ -556,29 +557,31 @@ __device__ __forceinline__ void mfma_stage_load(const BatchArgs& a, const BatchI
     c.p0 = p.p0;
     c.pds = p.pds;
     const int t2 = 2 * tid();
-    const int e2 = 2 * stage_pair(tid());   // the thread's first window pair
+    const int e4 = 4 * stage_quad(tid());   // the thread's first window quad
     if (o0 >= c.p0 && o0 + W <= int64_t(c.p0) + c.n) {
-        // All interior: a raw buffer over the window's W samples, each pair
-        // one 8-byte load at the lane's offset plus a scalar one (dword
+        // All interior: a raw buffer over the window's W samples, each quad
+        // one 16-byte load at the lane's offset plus a scalar one (dword
         // alignment is all a multi-dword load needs).  Lanes past the window
         // read zero, which the stage store never uses.
         const __amdgpu_buffer_rsrc_t wr = raw_buffer(a.x + c.x_off + (o0 - c.p0), W * int(sizeof(float)));
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b64(wr, e2 * int(sizeof(float)),
-                                                                u * 2 * T * int(sizeof(float)), 0);
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(wr, e4 * int(sizeof(float)),
+                                                                 u * 4 * T * int(sizeof(float)), 0);
             // copied out before the bit cast: clang's bit cast of a vector
             // element reads element 0 whatever the index
-            const uint32_t v0 = v[0], v1 = v[1];
-            st.w0[u] = __builtin_bit_cast(float, v0);
-            st.w1[u] = __builtin_bit_cast(float, v1);
+            const uint32_t v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+            st.w[u][0] = __builtin_bit_cast(float, v0);
+            st.w[u][1] = __builtin_bit_cast(float, v1);
+            st.w[u][2] = __builtin_bit_cast(float, v2);
+            st.w[u][3] = __builtin_bit_cast(float, v3);
         }
     } else {
 #pragma unroll
         for (int u = 0; u < MfmaStage::U; ++u) {
-            const int e = e2 + 2 * u * T;
-            st.w0[u] = *(e < W ? pad_ptr(a, c, o0 + e) : a.consts);
-            st.w1[u] = *(e < W ? pad_ptr(a, c, o0 + e + 1) : a.consts);
+            const int e = e4 + 4 * u * T;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) st.w[u][i] = *(e < W ? pad_ptr(a, c, o0 + e + i) : a.consts);
         }
     }
     taps3(a, p, kdil, t2 - 32, st.f0, st.f1, st.f2);
@@ -600,19 +603,21 @@ __device__ __forceinline__ void mfma_stage_store(const BatchArgs& a, const Batch
     const int W = kBatchMfmaTile + 16 * nkb + 32;
     const int ntp = 16 * nkb + 48;
     const int t2 = 2 * tid();
-    const int e2 = 2 * stage_pair(tid());
-    // Pair u sits at e = e2 + 2 u T, padded position padh(e2) + u (2 T) 5/4;
-    // pairs u < 8 are inside every window (W > kBatchMfmaTile).
-    static_assert(2 * T % 32 == 0 && 8 * 2 * T <= kBatchMfmaTile, "pair positions of the stage store");
-    const int h0 = padh(e2);
+    const int e4 = 4 * stage_quad(tid());
+    // Quad u sits at e = e4 + 4 u T, padded position padh(e4) + u (4 T) 5/4
+    // (8-byte aligned); quads u < 4 are inside every window (W > 4 (4 T)), and
+    // W is a multiple of 4, so a quad is inside or outside whole.
+    static_assert(4 * T % 32 == 0 && 4 * 4 * T <= kBatchMfmaTile, "quad positions of the stage store");
+    const int h0 = padh(e4);
 #pragma unroll
     for (int u = 0; u < MfmaStage::U; ++u) {
-        if (u < 8 || e2 + 2 * u * T < W) {
-            uint32_t w[P];
-            split_pair<P>(st.w0[u], st.w1[u], w);
+        if (u < 4 || e4 + 4 * u * T < W) {
+            uint32_t lo[P], hi[P];
+            split_pair<P>(st.w[u][0], st.w[u][1], lo);
+            split_pair<P>(st.w[u][2], st.w[u][3], hi);
 #pragma unroll
             for (int q = 0; q < P; ++q)
-                reinterpret_cast<uint32_t*>(wp + q * L::win_slots + h0 + u * (2 * T) * 5 / 4)[0] = w[q];
+                *reinterpret_cast<uint2*>(wp + q * L::win_slots + h0 + u * (4 * T) * 5 / 4) = make_uint2(lo[q], hi[q]);
         }
     }
     // pairs of both tap copies from three consecutive taps
