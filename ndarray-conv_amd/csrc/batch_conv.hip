@@ -662,6 +662,10 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);   // the queue's reads, on every lane
         return;
     }
+    // The stage and the prefetch run at a raised wave priority: the item's
+    // barrier waits for them, while the products of the other workgroups on
+    // the CU keep the matrix cores busy meanwhile.
+    __builtin_amdgcn_s_setprio(2);
     // two copies of the split: after fresh loads it waits for all of them,
     // after a prefetch only for loads older than the last item's stores
     if (have) {
@@ -674,6 +678,7 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
     if (have) mfma_stage_load(a, *next, st);
     advance();
     __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
     const int t = tid();
     const int lane = t & 63, wave = t >> 6;
     const int half = lane >> 5, m = lane & 31;
