@@ -6,8 +6,8 @@ One step is one convolution call over one batch of synthetic input already
 resident in HBM.  The default workload is the uneven batch: 4096 rank-1 f32
 signal/kernel pairs (signal lengths log-uniform in [256, 262144], kernel
 lengths log-uniform in [3, 1023], Uniform[-1, 1), ConvMode::Same,
-PaddingMode::Circular) in one persistent launch replayed through a HIP graph,
-the autotuner choosing BF16x3 / BF16x6 matrix-core direct, FP32 direct or
+PaddingMode::Circular) in one persistent launch (a HIP-graph replay of the
+same step is timed beside it), the autotuner choosing BF16x3 / BF16x6 matrix-core direct, FP32 direct or
 FP32 FFT per pair.  Other workloads (--workload): fft2d, fft2d_large, fft1d,
 fft3d, fft3d_c64, direct2d_i32, direct2d_f32 (with --algo bf16 / bf16x3), copy.
 
@@ -279,7 +279,8 @@ def batch_cpu_baseline(n, k, x, kv, threads, budget_s=10.0):
 
 def run_batch(args):
     """The uneven batch: 4096 rank-1 f32 pairs, one persistent launch per
-    step, replayed through a HIP graph."""
+    step (launched directly; a HIP-graph replay of the same step is timed
+    beside it and reported as graph_value)."""
     import torch
     import ndconv as nc
     from ndconv import _lib as C
@@ -310,9 +311,9 @@ def run_batch(args):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps, time.perf_counter() - t
 
-    note(f"batch plan ready ({plan_s:.1f} s); timing {args.steps} graph and eager steps")
-    ms_graph, wall = timed(bp, True, args.steps, args.warmup)
-    ms_eager, _ = timed(bp, False, args.steps, args.warmup)
+    note(f"batch plan ready ({plan_s:.1f} s); timing {args.steps} direct and graph-replayed steps")
+    ms_step, wall = timed(bp, False, args.steps, args.warmup)
+    ms_graph, _ = timed(bp, True, args.steps, args.warmup)
     outputs = int(bp.y_total)
     y_head = yt.cpu().numpy()
 
@@ -342,7 +343,7 @@ def run_batch(args):
     if args.batch_algo != "auto":
         note("timing the FP32-only form")
         bf = nc.BatchPlan(n, k, nc.ConvMode.Same, nc.PaddingMode.Circular, algo="auto", processor=proc)
-        fp32, _ = timed(bf, True, args.steps, args.warmup)
+        fp32, _ = timed(bf, False, args.steps, args.warmup)
         ys.append(yt.cpu().numpy())
         names.append("auto")
         bf.close()
@@ -363,27 +364,27 @@ def run_batch(args):
         note("CPU baseline, all threads then one")
         cpu = batch_cpu_baseline(n, k, x, kv, threads)
         cpu["single_thread"] = batch_cpu_baseline(n, k, x, kv, 1)
-    value = outputs / (ms_graph * 1e-3)
+    value = outputs / (ms_step * 1e-3)
     line = {"metric": METRIC, "value": value, "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms_graph, "higher_is_better": True, "scaling": "weak",
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "uneven batch: 4096 rank-1 f32 pairs, n log-uniform [256, 262144], "
                                    "k log-uniform [3, 1023], Uniform[-1,1), Same, Circular",
-                       "outputs_per_step": outputs, "seed": 5, "algo": args.batch_algo, "graph_replay": True},
+                       "outputs_per_step": outputs, "seed": 5, "algo": args.batch_algo, "graph_replay": False},
             "max_rel_err": errs[args.batch_algo]["max_rel_err"], "max_abs_err": errs[args.batch_algo]["max_abs_err"],
             "error_vs_fp64": errs,
             "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall,
-            "eager_value": outputs / (ms_eager * 1e-3), "ms_per_step_eager": ms_eager, "plan_s": plan_s,
+            "graph_value": outputs / (ms_graph * 1e-3), "ms_per_step_graph": ms_graph, "plan_s": plan_s,
             "schedule": {kk: report[kk] for kk in ("items", "lines", "kernel_spectra", "grid", "blocks_per_cu",
                                                    "costs")}}
     if fp32 is not None:
         line["fp32_auto_ms_per_step"] = fp32
         line["fp32_auto_value"] = outputs / (fp32 * 1e-3)
-        line["speedup_vs_fp32_auto"] = fp32 / ms_graph
+        line["speedup_vs_fp32_auto"] = fp32 / ms_step
     if plain is not None:
         line["plain_ms_per_step"] = plain
         line["plain_value"] = outputs / (plain * 1e-3)
-        line["speedup_vs_plain"] = plain / ms_graph
+        line["speedup_vs_plain"] = plain / ms_step
     print(json.dumps(line))
 
 
