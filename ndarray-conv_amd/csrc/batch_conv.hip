@@ -445,6 +445,7 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
     c.pds = p.pds;
     for (int t0 = 0; t0 < K; t0 += chunk) {
         const int nt = K - t0 < chunk ? K - t0 : chunk;
+        __builtin_amdgcn_s_setprio(2);   // staging, which the barrier below waits for, first
         const int64_t wbase = int64_t(o0) * s + int64_t(t0) * d;
         const int wlen = span + (nt - 1) * d;
         auto put = [&](int e, float v) { win[padw(e)] = v; };
@@ -456,6 +457,7 @@ __device__ __forceinline__ void direct_item(const BatchArgs& a, const BatchItem&
         }
         staged<4>(nt, [&](int t) { return corr_tap(a, p.k_off, K, t0 + t); }, [&](int t, float v) { taps[t] = v; });
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
         if (fast) {
             // 8 consecutive outputs per thread, a sliding register window
             const int ob = 8 * tid();
