@@ -407,9 +407,11 @@ __device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem&
     float* fb = reinterpret_cast<float*>(buf);
     // scattered kernel of the reference's FFT path (conv_fft/padding.rs:78-111):
     // k[i] at i d (reverse) or (K-1-i) d (no_reverse)
+    // u / d as the high word of 2u dmul (exact: u < 2 LC <= 8192, and a pair
+    // whose dilation passes 2^18 has one tap, whose only position is u = 0)
     load_reals<LC>(fb, lc, [&](const LineCtx& c, int u) -> const float* {
-        const int i = u / d;
-        if (!c.valid || u % d || i >= c.k) return a.consts;
+        const int i = int(__umulhi(uint32_t(u) << 1, a.dmul));
+        if (!c.valid || u != i * d || i >= c.k) return a.consts;
         return a.k + c.k_off + (a.rev ? i : c.k - 1 - i);
     });
     fft_lds<LC>(buf, q, false);

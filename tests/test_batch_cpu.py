@@ -65,10 +65,21 @@ def test_dilation_multiplier_divides_exactly():
     """Matrix-core items find tap q / d of a dilated kernel as the high word of
     2 q * ceil(2^31 / d) (batch.hpp, batch_dmul): exhaustively equal to q // d
     for every dilation up to 2^11 and every tap position an item reads
-    (q < 2^13)."""
+    (q < 2^13), and for sampled dilations up to 2^18 (the kernel-spectrum
+    pre-pass)."""
     q = np.arange(1 << 13, dtype=np.uint64)
     for d in range(1, (1 << 11) + 1):
         m = ((1 << 31) + d - 1) // d
         assert m < (1 << 32)
+        t = ((2 * q) * np.uint64(m)) >> np.uint64(32)
+        np.testing.assert_array_equal(t, q // np.uint64(d), err_msg=f"d={d}")
+    # the kernel-spectrum pre-pass uses it for every FFT pair's dilation
+    # (below 2^18 whenever the kernel has two taps or more) over positions
+    # below 8192: sampled dilations up to 2^18 and the edges
+    rng = np.random.default_rng(18)
+    extra = set(rng.integers((1 << 11) + 1, (1 << 18) + 1, 3000).tolist())
+    extra |= {(1 << 18), (1 << 18) - 1, 65521, 131071, 8191, 8192, 12289}
+    for d in sorted(extra):
+        m = ((1 << 31) + d - 1) // d
         t = ((2 * q) * np.uint64(m)) >> np.uint64(32)
         np.testing.assert_array_equal(t, q // np.uint64(d), err_msg=f"d={d}")
