@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -2342,6 +2343,181 @@ inline unsigned grid_for(int64_t total) {
     return unsigned(std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, int64_t(1) << 20)));
 }
 
+// Bluestein's chirp-z transform (L. I. Bluestein, "A linear filtering approach
+// to the computation of the discrete Fourier transform", 1970).  With
+// c[m] = exp(-i pi m^2 / N) (its conjugate for the inverse) and
+// j k = (j^2 + k^2 - (k - j)^2) / 2,
+//   X[k] = c[k] sum_j (x[j] c[j]) conj(c[k - j]),
+// a circular convolution of length M >= 2N - 1, a power of two whose
+// transforms split into LDS-resident factors.  Lines with no LDS-resident
+// split of their own (a large prime length, or twice one) take it.  Real
+// lines run as complex lines of their full length: r2c keeps bins 0..N/2;
+// c2r first rebuilds the Hermitian spectrum from them (the imaginary parts
+// of DC and, for even N, Nyquist ignored, as in the resident kernel).
+constexpr int64_t kBluesteinMaxM = int64_t(1) << 24;
+
+// In-place forward FFT of a power-of-two length, FP64 (iterative radix 2),
+// for the chirp's spectrum.
+void host_fft_pow2(std::vector<std::complex<double>>& v) {
+    const size_t M = v.size();
+    for (size_t i = 1, j = 0; i < M; ++i) {
+        size_t bit = M >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(v[i], v[j]);
+    }
+    std::vector<std::complex<double>> w(M / 2);
+    for (size_t k = 0; k < M / 2; ++k) {
+        const double a = -2.0 * M_PI * double(k) / double(M);
+        w[k] = {std::cos(a), std::sin(a)};
+    }
+    for (size_t len = 2; len <= M; len <<= 1) {
+        const size_t half = len / 2, step = M / len;
+        for (size_t b = 0; b < M; b += len)
+            for (size_t k = 0; k < half; ++k) {
+                const std::complex<double> t = v[b + k + half] * w[k * step];
+                v[b + k + half] = v[b + k] - t;
+                v[b + k] += t;
+            }
+    }
+}
+
+// Line l = o * inner + i of [outer][n][inner] times the chirp, zero-extended
+// to M points (M = 2^logM), as row l of dst.
+template <typename R>
+__global__ void __launch_bounds__(256) blue_in_kernel(const void* in, Cx<R>* dst, const Cx<R>* chirp, int64_t outer,
+                                                      int n, int64_t inner, int logM, int mode) {
+    using C = Cx<R>;
+    const int M = 1 << logM, H = n / 2 + 1;
+    const int64_t total = (outer * inner) << logM;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t l = e >> logM;
+        const int j = int(e & (M - 1));
+        C v = cmake<C>(R(0), R(0));
+        if (j < n) {
+            const int64_t o = l / inner, i = l - o * inner;
+            if (mode == 0) {
+                v = static_cast<const C*>(in)[(o * n + j) * inner + i];
+            } else if (mode == 1) {
+                v = cmake<C>(static_cast<const R*>(in)[o * n + j], R(0));
+            } else {
+                const C* X = static_cast<const C*>(in) + o * H;
+                v = j < H ? X[j] : cconj(X[n - j]);
+                if (j == 0 || 2 * j == n) v.y = R(0);
+            }
+            v = cmul(v, chirp[j]);
+        }
+        dst[e] = v;
+    }
+}
+
+// Row-wise product with the chirp's spectrum (which carries the 1/M of the
+// inverse transform).
+template <typename R>
+__global__ void __launch_bounds__(256) blue_mul_kernel(Cx<R>* buf, const Cx<R>* spec, int64_t lines, int logM) {
+    const int64_t total = lines << logM;
+    const int64_t mask = (int64_t(1) << logM) - 1;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x)
+        buf[e] = cmul(buf[e], spec[e & mask]);
+}
+
+// The first N (c2c, c2r) or N/2 + 1 (r2c) points of each row times the
+// chirp and `scale`, into the output's layout (c2r keeps real parts).
+template <typename R>
+__global__ void __launch_bounds__(256) blue_out_kernel(const Cx<R>* src, const Cx<R>* chirp, void* out, int64_t outer,
+                                                       int n, int64_t inner, int logM, int mode, R scale) {
+    using C = Cx<R>;
+    const int count = mode == 1 ? n / 2 + 1 : n;
+    const int64_t total = outer * inner * count;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t l = e / count;
+        const int k = int(e - l * count);
+        const C v = cscale(cmul(src[(l << logM) + k], chirp[k]), scale);
+        if (mode == 0) {
+            const int64_t o = l / inner, i = l - o * inner;
+            static_cast<C*>(out)[(o * n + k) * inner + i] = v;
+        } else if (mode == 1) {
+            static_cast<C*>(out)[e] = v;
+        } else {
+            static_cast<R*>(out)[l * n + k] = v.x;
+        }
+    }
+}
+
+template <typename R>
+ndc_status line_fft_bluestein(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
+                              int64_t inner, R scale) {
+    using C = Cx<R>;
+    constexpr bool kDbl = sizeof(R) == 8;
+    const size_t es = sizeof(C);
+    int logM = 0;
+    while ((int64_t(1) << logM) < 2 * int64_t(n) - 1) ++logM;
+    const int64_t M = int64_t(1) << logM;
+    if (M > kBluesteinMaxM) {
+        ctx.err = "line length has no LDS-resident split and exceeds the chirp-z transform's limit";
+        return NDC_ERR_UNSUPPORTED;
+    }
+    const bool backward = inv || mode == 2;
+    // chirp (n points) then the M-point spectrum of its conjugate, scaled by
+    // 1/M, made once per handle in FP64
+    const std::string key = std::string("bluestein:") + std::to_string(n) + (backward ? ":inv" : ":fwd") +
+                            (kDbl ? ":f64" : ":f32");
+    const size_t bytes = size_t(n + M) * es;
+    std::string& made = ctx.memo(key);
+    const void* tab = nullptr;
+    if (made.empty() && !ctx.dry_run()) {
+        std::vector<std::complex<double>> c(static_cast<size_t>(n)), b(static_cast<size_t>(M));
+        for (int64_t m = 0; m < n; ++m) {
+            const double ang = M_PI * double((m * m) % (2 * int64_t(n))) / double(n);
+            c[size_t(m)] = {std::cos(ang), backward ? std::sin(ang) : -std::sin(ang)};
+            b[size_t(m)] = std::conj(c[size_t(m)]);
+            if (m > 0) b[size_t(M - m)] = std::conj(c[size_t(m)]);
+        }
+        host_fft_pow2(b);
+        std::vector<R> host(size_t(n + M) * 2);
+        for (int64_t m = 0; m < n; ++m) {
+            host[size_t(2 * m)] = R(c[size_t(m)].real());
+            host[size_t(2 * m + 1)] = R(c[size_t(m)].imag());
+        }
+        for (int64_t j = 0; j < M; ++j) {
+            host[size_t(2 * (n + j))] = R(b[size_t(j)].real() / double(M));
+            host[size_t(2 * (n + j) + 1)] = R(b[size_t(j)].imag() / double(M));
+        }
+        tab = ctx.constant_table(key, host.data(), bytes);
+        if (tab) made = "1";
+    } else {
+        tab = ctx.constant_table(key, nullptr, bytes);
+    }
+    const int64_t lines = outer * inner;
+    C* buf = static_cast<C*>(ctx.scratch(3, size_t(lines * M) * es));
+    if (!tab || !buf) {
+        ctx.err = "chirp-z workspace allocation failed";
+        return NDC_ERR_OUT_OF_MEMORY;
+    }
+    const C* chirp = static_cast<const C*>(tab);
+    const C* spec = chirp + n;
+    ndc_status s = NDC_OK;
+    const double io_in = double(lines) * (double(n) * (mode == 1 ? sizeof(R) : es) + double(M) * es);
+    ctx.kbegin("fft_processor_chirp_io", io_in, 0);
+    NDC_LAUNCH(ctx, ctx.stream(), (blue_in_kernel<R>), dim3(grid_for(lines * M)), dim3(256), 0, in, buf, chirp, outer,
+               n, inner, logM, mode);
+    ctx.kend();
+    if ((s = check_launch(ctx, "chirp-z input")) != NDC_OK) return s;
+    if ((s = line_fft<R>(ctx, 0, false, buf, buf, lines, int(M), 1, R(1))) != NDC_OK) return s;
+    ctx.kbegin("fft_processor_chirp_io", 2.0 * double(lines * M) * es, 0);
+    NDC_LAUNCH(ctx, ctx.stream(), (blue_mul_kernel<R>), dim3(grid_for(lines * M)), dim3(256), 0, buf, spec, lines,
+               logM);
+    ctx.kend();
+    if ((s = check_launch(ctx, "chirp-z product")) != NDC_OK) return s;
+    if ((s = line_fft<R>(ctx, 0, true, buf, buf, lines, int(M), 1, R(1))) != NDC_OK) return s;
+    const int64_t count = mode == 1 ? n / 2 + 1 : n;
+    ctx.kbegin("fft_processor_chirp_io", double(lines * count) * (es + (mode == 2 ? sizeof(R) : es)), 0);
+    NDC_LAUNCH(ctx, ctx.stream(), (blue_out_kernel<R>), dim3(grid_for(lines * count)), dim3(256), 0, buf, chirp, out,
+               outer, n, inner, logM, mode, scale);
+    ctx.kend();
+    return check_launch(ctx, "chirp-z output");
+}
+
 // A line of n points that does not fit one workgroup: n (the complex length
 // for c2c and odd real lines, n / 2 for even real ones) = L1 * L2 with both
 // factors LDS-resident.  Pass A transforms L1-point lines along j1 (L2 and the
@@ -2369,10 +2545,7 @@ ndc_status line_fft_long(LaunchContext& ctx, int mode, bool inv, const void* in,
     int64_t L2 = 0;
     for (int64_t l2 = 2; l2 * l2 <= Lc; ++l2)
         if (Lc % l2 == 0 && fits(l2) && fits(Lc / l2)) L2 = l2;
-    if (!L2) {
-        ctx.err = "line length cannot be split into LDS-resident factors";
-        return NDC_ERR_UNSUPPORTED;
-    }
+    if (!L2) return line_fft_bluestein<R>(ctx, mode, inv, in, out, outer, n, inner, scale);
     const int64_t L1 = Lc / L2;
     const C* twc = static_cast<const C*>(ctx.twiddles(Lc, kDbl));
     const C* twn = half ? static_cast<const C*>(ctx.twiddles(n, kDbl)) : twc;

@@ -3,12 +3,16 @@ backward, processor/mod.rs:79-121) against the oracle's restatement of the
 reference's layout: the N-d DFT with the axes rotated left by one, real input
 keeping F/2+1 bins of its last axis; backward returns natural order divided by
 the element count."""
+import ctypes
+import json
+
 import numpy as np
 import pytest
 
 from oracle import ndconv_oracle as O
 
 import ndconv as nc
+from ndconv import _lib as C
 
 pytestmark = pytest.mark.gpu
 
@@ -113,8 +117,42 @@ def test_long_lines(gpu, shape, dt):
     p.close()
 
 
-def test_long_prime_line_is_refused(gpu):
-    # a prime length beyond one workgroup's LDS has no LDS-resident split
+# Lengths with no LDS-resident split (a prime beyond one workgroup's LDS, or
+# twice one for a real line's half-length packing) take the chirp-z form:
+# complex lines, strided columns, odd and even real lines, both directions.
+CHIRP = [((10007,), np.complex64), ((10007,), np.complex128), ((3, 10007), np.float32), ((2, 20014), np.float64),
+         ((10007, 3), np.complex128), ((19946,), np.complex64), ((4, 4801), np.float64), ((65537,), np.float32)]
+
+
+@pytest.mark.parametrize("shape,dt", CHIRP, ids=lambda v: str(v))
+def test_lines_without_a_split(gpu, shape, dt):
+    rng = np.random.default_rng(sum(shape) + 1)
+    x = make(rng, shape, dt)
+    p = nc.get_fft_processor(dt)
+    for rep in range(2):   # the chirp tables are made on the first call and reused
+        if rep == 0:
+            lib = C.lib()
+            nc._check(lib.ndc_profile_enable(p.handle, 1), p.handle)
+            spec = p.forward(x)
+            buf = ctypes.create_string_buffer(1 << 16)
+            nc._check(lib.ndc_profile_report(p.handle, buf, len(buf)), p.handle)
+            nc._check(lib.ndc_profile_enable(p.handle, 0), p.handle)
+            assert "fft_processor_chirp_io" in json.loads(buf.value.decode()), shape   # the chirp-z form ran
+        else:
+            spec = p.forward(x)
+        wide = np.complex128 if np.dtype(dt).kind == "c" else np.float64
+        ref = O.processor_forward(x.astype(wide))
+        assert spec.shape == ref.shape
+        tol = TOL[np.dtype(dt)] * 10
+        assert O.relerr(spec, ref)[0] <= tol, (shape, rep)
+        back = p.backward(spec)
+        assert back.shape == x.shape and back.dtype == x.dtype
+        assert O.relerr(back, x)[0] <= 5 * tol, (shape, rep)
+    p.close()
+
+
+def test_chirp_length_limit(gpu):
+    # a prime beyond 2^23 would need a chirp-z convolution longer than 2^24
     p = nc.get_fft_processor(np.complex64)
     with pytest.raises(nc.Unsupported):
-        p.forward(np.zeros(10007, np.complex64))
+        p.forward(np.zeros(8388617, np.complex64))
