@@ -255,8 +255,9 @@ ndc_status ndc_baked_destroy(ndc_baked b);
  *    backward restores the last axis to the recorded length (the imaginary parts
  *    of the DC and Nyquist bins are ignored, as realfft's result is used there).
  * Lines longer than one workgroup's LDS holds are split in two LDS-resident
- * factors (lengths whose factors do not fit, e.g. a prime above about 9000
- * points, return NDC_ERR_UNSUPPORTED). */
+ * factors; lengths with no such split (e.g. a prime above about 9000 points)
+ * run as Bluestein's chirp-z transform, up to 2^23 points (longer ones return
+ * NDC_ERR_UNSUPPORTED). */
 ndc_status ndc_fft_forward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* shape, const void* x, void* y,
                            uint32_t flags);
 ndc_status ndc_fft_backward(ndc_handle h, int32_t dtype, int32_t rank, const int64_t* spec_shape, const void* spec,
