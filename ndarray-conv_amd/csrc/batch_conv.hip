@@ -786,8 +786,23 @@ __device__ __forceinline__ void mfma_item(const BatchArgs& a, const BatchItem& p
 static_assert(MfmaLds<2>::bytes <= size_t(kDataSlots) * sizeof(C), "BF16x3 staging must fit the FFT data region");
 static_assert(MfmaLds<3>::bytes <= kLdsBytes, "BF16x6 staging must fit the dynamic LDS");
 
+// A class's quarter table (n = LC/2 <= kTwSlots points): every load of a
+// thread is issued before its first LDS store.  A plain copy loop waits for
+// each load before the next, eight serial round trips for the largest class.
 __device__ __forceinline__ void stage_tw(C* dst, const C* src, int n) {
-    for (int i = tid(); i < n; i += T) dst[i] = src[i];
+    constexpr int U = kTwSlots / T;
+    static_assert(kTwSlots % T == 0 && batch_lc(kBatchClasses - 1) / 2 <= kTwSlots, "one round per thread slot");
+    C v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = tid() + u * T;
+        if (i < n) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int i = tid() + u * T;
+        if (i < n) dst[i] = v[u];
+    }
 }
 
 __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a) {
