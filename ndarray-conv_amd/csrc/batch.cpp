@@ -293,7 +293,7 @@ Schedule build_schedule(const std::vector<PairShape>& sh, const std::vector<std:
 
 // Device-resident copy of a schedule plus the launch arguments.
 struct DevSchedule {
-    DevMem pairs, items, lines, kpairs, kitems, kspec, counter, consts;
+    DevMem pairs, items, lines, klines, kitems, kspec, counter, consts;
     BatchArgs args{};
     int grid = 0;
     // Border sides of every pair, and the constant cells the kernels read for
@@ -309,17 +309,22 @@ struct DevSchedule {
         return true;
     }
     bool upload(const Schedule& S, const float2* tw, int blocks) {
+        std::vector<BatchKLine> kl(S.kpairs.size());
+        for (size_t i = 0; i < kl.size(); ++i) {
+            const BatchPair& p = S.pairs[size_t(S.kpairs[i])];
+            kl[i] = BatchKLine{p.k_off, p.ks_off, p.k, 0};
+        }
         if (!pairs.upload(S.pairs.data(), S.pairs.size() * sizeof(BatchPair)) ||
             !items.upload(S.items.data(), S.items.size() * sizeof(BatchItem)) ||
             !lines.upload(S.lines.data(), S.lines.size() * sizeof(BatchLine)) ||
-            !kpairs.upload(S.kpairs.data(), S.kpairs.size() * sizeof(int32_t)) ||
+            !klines.upload(kl.data(), kl.size() * sizeof(BatchKLine)) ||
             !kitems.upload(S.kitems.data(), S.kitems.size() * sizeof(BatchKItem)) ||
             !kspec.ensure(size_t(S.ks_total) * sizeof(float2)) || !counter.ensure(size_t(kBatchHeads) * kBatchQueueStride * sizeof(int32_t)))
             return false;
         args.pairs = static_cast<const BatchPair*>(pairs.p);
         args.items = static_cast<const BatchItem*>(items.p);
         args.lines = static_cast<const BatchLine*>(lines.p);
-        args.kpairs = static_cast<const int32_t*>(kpairs.p);
+        args.klines = static_cast<const BatchKLine*>(klines.p);
         args.kitems = static_cast<const BatchKItem*>(kitems.p);
         args.kspec = static_cast<float2*>(kspec.p);
         args.counter = static_cast<int32_t*>(counter.p);

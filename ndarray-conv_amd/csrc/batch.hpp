@@ -105,9 +105,16 @@ struct BatchLine {
     int32_t n, k, p0, pds, out, per, o0, pair;
 };
 
-// Kernel-spectrum pre-pass work: up to batch_lines(cls) pairs of one class.
+// Kernel-spectrum pre-pass work: up to batch_lines(cls) pairs of one class,
+// klines [first, first + count).
 struct BatchKItem {
     int32_t cls, first, count, pad_;
+};
+
+// What the pre-pass reads of a pair, in pre-pass order (24 bytes, one load).
+struct BatchKLine {
+    int64_t k_off, ks_off;         // kernel offset into k, complex offset of its spectrum
+    int32_t k, pad_;               // kernel length
 };
 
 // Shared launch parameters of a batch.
@@ -118,7 +125,7 @@ struct BatchArgs {
     const BatchPair* pairs;
     const BatchItem* items;
     const BatchLine* lines;
-    const int32_t* kpairs;         // pairs of the kernel-spectrum pre-pass, grouped by class
+    const BatchKLine* klines;      // pairs of the kernel-spectrum pre-pass, grouped by class
     const BatchKItem* kitems;
     float2* kspec;
     int32_t* counter;              // kBatchHeads sub-queue heads (kBatchQueueStride apart), reset by the pre-pass

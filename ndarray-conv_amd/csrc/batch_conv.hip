@@ -247,42 +247,26 @@ __device__ __forceinline__ float corr_tap(const BatchArgs& a, int64_t k_off, int
 }
 
 // span: window length in samples (the interior test covers [o0 s, o0 s + span)).
-__device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, int G, int first, int count,
-                                              bool kernel_lines, int span) {
+__device__ __forceinline__ void load_line_ctx(const BatchArgs& a, LineCtx* lc, int G, int first, int count, int span) {
     const int g = tid();
     if (g < G) {
         LineCtx c{};
         c.valid = g < count;
         if (c.valid) {
-            if (kernel_lines) {
-                const BatchPair p = a.pairs[a.kpairs[first + g]];
-                c.o0 = 0;
-                c.x_off = p.x_off;
-                c.y_off = p.y_off;
-                c.ks_off = p.ks_off;
-                c.k_off = p.k_off;
-                c.n = p.n;
-                c.k = p.k;
-                c.p0 = p.p0;
-                c.pds = p.pds;
-                c.out = p.out;
-                c.per = p.per;
-            } else {
-                // a data line's record carries its pair's metadata: one load
-                const BatchLine ln = a.lines[first + g];
-                c.o0 = ln.o0;
-                c.x_off = ln.x_off;
-                c.y_off = ln.y_off;
-                c.ks_off = ln.ks_off;
-                c.n = ln.n;
-                c.k = ln.k;
-                c.p0 = ln.p0;
-                c.pds = ln.pds;
-                c.out = ln.out;
-                c.per = ln.per;
-            }
+            // a data line's record carries its pair's metadata: one load
+            const BatchLine ln = a.lines[first + g];
+            c.o0 = ln.o0;
+            c.x_off = ln.x_off;
+            c.y_off = ln.y_off;
+            c.ks_off = ln.ks_off;
+            c.n = ln.n;
+            c.k = ln.k;
+            c.p0 = ln.p0;
+            c.pds = ln.pds;
+            c.out = ln.out;
+            c.per = ln.per;
             const int64_t P0 = int64_t(c.o0) * a.s;
-            c.interior = !kernel_lines && P0 >= c.p0 && P0 + span <= int64_t(c.p0) + c.n;
+            c.interior = P0 >= c.p0 && P0 + span <= int64_t(c.p0) + c.n;
             c.src = c.x_off + P0 - c.p0;
         }
         lc[g] = c;
@@ -319,7 +303,7 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
     constexpr int LS = LC + LC / 16;
     constexpr int L = 2 * LC;
     constexpr int H = LC / 2 + 1;
-    load_line_ctx(a, lc, G, it.a, it.b, false, L);
+    load_line_ctx(a, lc, G, it.a, it.b, L);
     __syncthreads();
     const int s = a.s;
     float* fb = reinterpret_cast<float*>(buf);
@@ -396,13 +380,13 @@ __device__ __forceinline__ void fft_item(const BatchArgs& a, const BatchItem& it
     }
 }
 
+// The lines' context (valid, k, k_off, ks_off) is in lc, and the class's
+// twiddle table in q.
 template <int LC>
-__device__ __forceinline__ void kspec_item(const BatchArgs& a, const BatchKItem& it, const C* q, C* buf, LineCtx* lc) {
+__device__ __forceinline__ void kspec_item(const BatchArgs& a, const C* q, C* buf, const LineCtx* lc) {
     constexpr int G = kBatchPoints / LC;
     constexpr int LS = LC + LC / 16;
     constexpr int L = 2 * LC;
-    load_line_ctx(a, lc, G, it.first, it.count, true, 0);
-    __syncthreads();
     const int d = a.d;
     float* fb = reinterpret_cast<float*>(buf);
     // scattered kernel of the reference's FFT path (conv_fft/padding.rs:78-111):
@@ -814,14 +798,28 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kspec_kernel(BatchArgs a)
     C* q = smem;
     C* buf = smem + kTwSlots;
     const int LC = batch_lc(it.cls);
+    const int G = kBatchPoints / LC;
+    // the lines' records and the twiddle table in flight together
+    const int g = tid();
+    const bool valid = g < G && g < it.count;
+    BatchKLine kl{};
+    if (valid) kl = a.klines[it.first + g];
     stage_tw(q, a.tw + batch_tw_offset(it.cls), LC / 2);
+    if (g < G) {
+        LineCtx c{};
+        c.valid = valid;
+        c.k_off = kl.k_off;
+        c.ks_off = kl.ks_off;
+        c.k = kl.k;
+        lc[g] = c;
+    }
     __syncthreads();
     switch (it.cls) {
-        case 0: kspec_item<256>(a, it, q, buf, lc); break;
-        case 1: kspec_item<512>(a, it, q, buf, lc); break;
-        case 2: kspec_item<1024>(a, it, q, buf, lc); break;
-        case 3: kspec_item<2048>(a, it, q, buf, lc); break;
-        default: kspec_item<4096>(a, it, q, buf, lc); break;
+        case 0: kspec_item<256>(a, q, buf, lc); break;
+        case 1: kspec_item<512>(a, q, buf, lc); break;
+        case 2: kspec_item<1024>(a, q, buf, lc); break;
+        case 3: kspec_item<2048>(a, q, buf, lc); break;
+        default: kspec_item<4096>(a, q, buf, lc); break;
     }
 }
 
@@ -862,7 +860,7 @@ __device__ __forceinline__ BatchArgs kernel_args(KernargArgs arg) {
     a.pairs = ap->pairs;
     a.items = ap->items;
     a.lines = ap->lines;
-    a.kpairs = ap->kpairs;
+    a.klines = ap->klines;
     a.kitems = ap->kitems;
     a.kspec = ap->kspec;
     a.counter = ap->counter;
