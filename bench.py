@@ -396,6 +396,9 @@ def main():
     ap.add_argument("--workload", default="batch1d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="also time hipGraph replay of the step")
+    ap.add_argument("--baked", action="store_true",
+                    help="conv_fft workloads: the kernel's spectrum made once before timing (ndc_conv_fft_bake), "
+                         "each step ndc_conv_fft_with_baked")
     ap.add_argument("--algo", default="auto", help="direct workloads: auto (exact FP32/integer), bf16, bf16x3")
     ap.add_argument("--batch-algo", default="auto_bf16x3",
                     help="uneven batch: auto_bf16x3 (autotuned matrix-core/FP32 mix), auto (FP32 only), fft, direct, "
@@ -438,9 +441,15 @@ def main():
     nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
     px, pk, py = ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(kt.data_ptr()), ctypes.c_void_p(yt.data_ptr())
     dref = ctypes.byref(d)
+    baked = None
+    if args.baked:
+        if op != C.OP_CONV_FFT:
+            sys.exit("--baked applies to conv_fft workloads")
+        baked = ctypes.c_void_p()
+        nc._check(lib.ndc_conv_fft_bake(h, dref, pk, ctypes.byref(baked)), h)
 
     def step():
-        st = fn(h, dref, px, pk, py)
+        st = lib.ndc_conv_fft_with_baked(baked, px, py) if baked is not None else fn(h, dref, px, pk, py)
         if st:
             nc._check(st, h)
 
@@ -518,6 +527,8 @@ def main():
         nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
 
     y_host = yt.cpu().numpy()
+    if baked is not None:
+        nc._check(lib.ndc_baked_destroy(baked), h)
     note("checking against the oracle")
     ref = oracle_result(w, x, k)
     if np.dtype(w["dtype"]).kind in "iu":
@@ -533,8 +544,8 @@ def main():
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": {np.dtype(np.float32): "f32", np.dtype(np.complex64): "c64",
                                            np.dtype(np.int32): "i32"}[np.dtype(w["dtype"])],
-            "data": "synthetic", "config": {"workload": w["name"], "outputs_per_step": outputs,
-                                            "seed": w["seed"], "algo": args.algo},
+            "data": "synthetic", "config": {"workload": w["name"] + (", kernel spectrum baked" if baked is not None else ""),
+                                            "outputs_per_step": outputs, "seed": w["seed"], "algo": args.algo},
             **acc, "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall}
     if graph_value is not None:
         line["graph_replay_value"] = graph_value

@@ -1109,24 +1109,47 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
         other = D;          // free once the product is formed
     }
     const C* wpost = tw + a.pl.post;
+    // With a stored kernel spectrum, a thread's bins of it come in rounds of
+    // kU, every load of a round issued before the round's first product (a
+    // load per bin inside the loop would wait for each in turn).
+    auto ks_rounds = [&](int n, auto&& get, auto&& use) {
+        for (int base = threadIdx.x; base < n; base += kU * T) {
+            C v0[kU], v1[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int k = base + u * T;
+                v0[u] = v1[u] = cmake<C>(R(0), R(0));
+                if (k < n) get(k, v0[u], v1[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (base + u * T < n) use(base + u * T, v0[u], v1[u]);
+        }
+    };
     if (a.mode == 0) {
         // bins k and M-k together: forward post-process both spectra,
         // multiply, then the c2r pre-process of the product
         const int M = L;
-        for (int k = threadIdx.x; k <= M / 2; k += T) {
+        auto product = [&](int k, C Kk, C Km) {
             const int km = M - k;
-            C Pk = cmul(r2c_bin(D, M, k, wpost, des), a.ks ? a.ks[k] : r2c_bin(K, M, k, wpost, des));
-            C Pm = cmul(r2c_bin(D, M, km, wpost, des), a.ks ? a.ks[km] : r2c_bin(K, M, km, wpost, des));
+            C Pk = cmul(r2c_bin(D, M, k, wpost, des), Kk);
+            C Pm = cmul(r2c_bin(D, M, km, wpost, des), Km);
             if (k == 0) { Pk.y = 0; Pm.y = 0; }   // Im of DC and Nyquist ignored by c2r
             W[pp(k == M ? 0 : k)] = c2r_pre(Pk, cconj(Pm), k, wpost);
             if (km != k && km < M) W[pp(km)] = c2r_pre(Pm, cconj(Pk), km, wpost);
+        };
+        if (a.ks) {
+            ks_rounds(M / 2 + 1, [&](int k, C& k0, C& k1) { k0 = a.ks[k]; k1 = a.ks[M - k]; }, product);
+        } else {
+            for (int k = threadIdx.x; k <= M / 2; k += T)
+                product(k, r2c_bin(K, M, k, wpost, des), r2c_bin(K, M, M - k, wpost, des));
         }
     } else if (a.mode == 1) {
         // odd F: the product of two Hermitian spectra, used whole
         if (a.ks) {
             // the stored kernel spectrum holds bins 0..F/2; the rest by symmetry
-            for (int k = threadIdx.x; k < L; k += T)
-                W[pp(k)] = cmul(D[pp(k)], k < a.H ? a.ks[k] : cconj(a.ks[L - k]));
+            ks_rounds(L, [&](int k, C& k0, C&) { k0 = k < a.H ? a.ks[k] : cconj(a.ks[L - k]); },
+                      [&](int k, C k0, C) { W[pp(k)] = cmul(D[pp(k)], k0); });
         } else {
             for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k) * des], K[pp(k) * des]);
         }
@@ -1134,7 +1157,7 @@ __global__ void __launch_bounds__(kMaxThreads) fused1d_kernel(Fused1dArgs<R, E> 
     } else if (a.ks) {
         // separate loops: a select between the global and the LDS pointer
         // would turn both into flat accesses
-        for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k)], a.ks[k]);
+        ks_rounds(L, [&](int k, C& k0, C&) { k0 = a.ks[k]; }, [&](int k, C k0, C) { W[pp(k)] = cmul(D[pp(k)], k0); });
     } else {
         for (int k = threadIdx.x; k < L; k += T) W[pp(k)] = cmul(D[pp(k) * des], K[pp(k) * des]);
     }
