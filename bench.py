@@ -9,7 +9,9 @@ lengths log-uniform in [3, 1023], Uniform[-1, 1), ConvMode::Same,
 PaddingMode::Circular) in one persistent launch (a HIP-graph replay of the
 same step is timed beside it), the autotuner choosing BF16x3 / BF16x6 matrix-core direct, FP32 direct or
 FP32 FFT per pair.  Other workloads (--workload): fft2d, fft2d_large, fft1d,
-fft3d, fft3d_c64, direct2d_i32, direct2d_f32 (with --algo bf16 / bf16x3), copy.
+fft3d, fft3d_c64, direct2d_i32, direct2d_f32 (with --algo bf16 / bf16x3), proc2d (the FFT
+processor's forward + backward round trip of a 256x5120 f32 array), copy.  The conv_fft
+workloads take --baked (the kernel spectrum made once before timing).
 
 Prints ONE JSON line with the throughput, the error against the FP64 oracle
 (normwise relative and max abs; bit-exactness for integers), the roofline of
@@ -277,6 +279,88 @@ def batch_cpu_baseline(n, k, x, kv, threads, budget_s=10.0):
             "max_rel_err": rel, "max_abs_err": ab}
 
 
+def run_proc(args):
+    """FftProcessor (processor/mod.rs:79-121, real.rs:105-155,233-281): one step is
+    forward then backward of a 256x5120 f32 array on the device, the processor
+    shape of the 2-D configuration."""
+    import torch
+    import ndconv as nc
+    from ndconv import _lib as C
+    from oracle import ndconv_oracle as O
+
+    shape = (256, 5120)
+    spec_shape = (shape[1] // 2 + 1, shape[0])
+    x = np.random.default_rng(7).standard_normal(shape).astype(np.float32)
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(x).to(dev)
+    st = torch.empty(spec_shape, dtype=torch.complex64, device=dev)
+    bt = torch.empty(shape, dtype=torch.float32, device=dev)
+    lib = C.lib()
+    proc = nc.get_fft_processor(np.float32)
+    h = proc.handle
+    stream = torch.cuda.current_stream(dev)
+    nc._check(lib.ndc_set_stream(h, ctypes.c_void_p(stream.cuda_stream)), h)
+    sh = (ctypes.c_int64 * 2)(*shape)
+    ssh = (ctypes.c_int64 * 2)(*spec_shape)
+    px, ps, pb = ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(st.data_ptr()), ctypes.c_void_p(bt.data_ptr())
+
+    def step():
+        nc._check(lib.ndc_fft_forward(h, C.F32, 2, sh, px, ps, 0), h)
+        nc._check(lib.ndc_fft_backward(h, C.F32, 2, ssh, ps, pb, 0), h)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    outputs = int(np.prod(shape))
+    nc._check(lib.ndc_profile_enable(h, 1), h)
+    for _ in range(args.steps):
+        step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    nc._check(lib.ndc_profile_report(h, buf, len(buf)), h)
+    nc._check(lib.ndc_profile_enable(h, 0), h)
+    prof = json.loads(buf.value.decode())
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bpl = dom["bytes"] / dom["launches"]
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "kernel": dom_name, "avg_ms": avg_ms, "bytes_per_launch": bpl}
+    kernels = {n: {"avg_ms": v["total_ms"] / v["launches"], "launches_per_step": v["launches"] / args.steps,
+                   "GB/s": (v["bytes"] / v["launches"]) / (v["total_ms"] / v["launches"] * 1e-3) / 1e9}
+               for n, v in prof.items()}
+    note("checking against the oracle")
+    ref = O.processor_forward(x.astype(np.float64))
+    fwd_err = O.relerr(st.cpu().numpy(), ref)
+    rt_err = O.relerr(bt.cpu().numpy(), x)
+    cpu = None
+    if not args.no_cpu:
+        note("CPU baseline")
+        reps, t = 0, time.perf_counter()
+        while time.perf_counter() - t < 10.0 and reps < 200:
+            O.processor_backward(O.processor_forward(x), shape[1])
+            reps += 1
+        cpu_s = (time.perf_counter() - t) / reps
+        cpu = {"value": outputs / cpu_s, "unit": "outputs/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} round trips of the whole array: the oracle's numpy restatement (FP64 pocketfft), one thread"}
+    line = {"metric": METRIC, "value": outputs / (ms * 1e-3), "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "FftProcessor forward + backward, 256x5120 f32 (spectrum 2561x256 c64)",
+                       "outputs_per_step": outputs, "seed": 7},
+            "forward_max_rel_err": fwd_err[0], "roundtrip_max_rel_err": rt_err[0], "roofline": roof,
+            "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall}
+    print(json.dumps(line))
+
+
 def run_batch(args):
     """The uneven batch: 4096 rank-1 f32 pairs, one persistent launch per
     step (launched directly; a HIP-graph replay of the same step is timed
@@ -413,6 +497,9 @@ def main():
         return
     if args.workload == "copy":
         run_copy(args)
+        return
+    if args.workload == "proc2d":
+        run_proc(args)
         return
 
     import torch
