@@ -2043,6 +2043,11 @@ struct LineFftArgs {
     // W^(k1 j2) of twN (W_nfull^e, e < n_full; conjugated for the inverse)
     const Cx<R>* twN;
     int64_t tw_inner;
+    // columns only: the input (in_lines) or output (out_lines) holds each
+    // line contiguously, element (o, j, i) at ((o inner + i) n + j): the
+    // processor's axis rotation fused into its first inverse or last forward
+    // transform instead of a transpose pass
+    int32_t in_lines, out_lines;
     LinePlan<R> pl;      // complex length transformed: F/2 for even real lines, else n
 };
 
@@ -2075,7 +2080,8 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
                 v[u] = cmake<C>(R(0), R(0));
                 if (it < L * Ct) {
                     const int j = fdiv(it, Ct, rC), c = it - j * Ct;
-                    if (c < nc) v[u] = a.in[base + int64_t(j) * a.inner + c];
+                    if (c < nc)
+                        v[u] = a.in_lines ? a.in[(o * a.inner + i0 + c) * a.n + j] : a.in[base + int64_t(j) * a.inner + c];
                 }
             }
 #pragma unroll
@@ -2098,7 +2104,8 @@ __global__ void __launch_bounds__(kMaxThreads) line_fft_kernel(LineFftArgs<R> a)
                 const C w = a.twN[int64_t(j) * j2];
                 v = cmul(v, inv ? cconj(w) : w);
             }
-            a.out[base + int64_t(j) * a.inner + c] = v;
+            if (a.out_lines) a.out[(o * a.inner + i0 + c) * a.n + j] = v;
+            else a.out[base + int64_t(j) * a.inner + c] = v;
         }
         return;
     }
@@ -2297,18 +2304,34 @@ template <typename R>
 ndc_status line_fft_long(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
                          int64_t inner, R scale);
 
+// Whether a line of n points (mode as in line_fft) is transformed by one
+// workgroup in LDS (otherwise line_fft takes the split form).
+template <typename R>
+bool line_fits_lds(LaunchContext& ctx, int mode, int n) {
+    const bool half = mode != 0 && n % 2 == 0;
+    const int L = half ? n / 2 : n;
+    return (size_t(ntw_of(L, half ? n : 0, sizeof(R) == 8)) + size_t(2) * pp_len(L)) * sizeof(Cx<R>) <= ctx.max_lds();
+}
+
+constexpr int kInLines = 1, kOutLines = 2;   // line_fft io: see LineFftArgs::in_lines / out_lines
+
 // One line-transform launch along an axis of [outer][n][inner]; lines that do
 // not fit one workgroup's LDS take the split form (line_fft_long).  twN and
-// tw_inner: the first pass of such a split (see LineFftArgs).
+// tw_inner: the first pass of such a split (see LineFftArgs).  io: kInLines /
+// kOutLines for LDS-resident lines (see LineFftArgs).
 template <typename R>
 ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void* out, int64_t outer, int n,
-                    int64_t inner, R scale, const Cx<R>* twN = nullptr, int64_t tw_inner = 1) {
+                    int64_t inner, R scale, const Cx<R>* twN = nullptr, int64_t tw_inner = 1, int io = 0) {
     using C = Cx<R>;
     LineFftArgs<R> a;
     std::memset(&a, 0, sizeof(a));
     const bool half = mode != 0 && n % 2 == 0;
     const int L = half ? n / 2 : n;
-    if ((size_t(ntw_of(L, half ? n : 0, sizeof(R) == 8)) + size_t(2) * pp_len(L)) * sizeof(C) > ctx.max_lds()) {
+    if (!line_fits_lds<R>(ctx, mode, n)) {
+        if (io) {
+            ctx.err = "line-contiguous input or output needs an LDS-resident line";
+            return NDC_ERR_UNSUPPORTED;
+        }
         if (twN) {
             ctx.err = "split line factor exceeds the LDS transform size";
             return NDC_ERR_UNSUPPORTED;
@@ -2332,6 +2355,8 @@ ndc_status line_fft(LaunchContext& ctx, int mode, bool inv, const void* in, void
     a.scale = scale;
     a.twN = twN;
     a.tw_inner = tw_inner;
+    a.in_lines = (io & kInLines) ? 1 : 0;
+    a.out_lines = (io & kOutLines) ? 1 : 0;
     const size_t es = sizeof(C);
     const size_t lds = ctx.max_lds();
     int Ct = 1;
@@ -2659,14 +2684,20 @@ ndc_status proc_forward(bool real, int N, const int64_t* shape, const void* in, 
     ndc_status s = line_fft<R>(ctx, real ? 1 : 0, false, in, work, rows, int(Fl), 1, R(1));
     if (s != NDC_OK) return s;
     int64_t inner = H;
+    // the last (axis 0) transform writes the rotated layout itself when its
+    // lines fit LDS: line i of [F0][inner] lands at out[i F0 ...]
+    const bool fused = N > 1 && line_fits_lds<R>(ctx, 0, int(shape[0]));
     for (int ax = N - 2; ax >= 0; --ax) {
         int64_t outer = 1;
         for (int i = 0; i < ax; ++i) outer *= shape[i];
-        s = line_fft<R>(ctx, 0, false, work, work, outer, int(shape[ax]), inner, R(1));
+        if (ax == 0 && fused)
+            s = line_fft<R>(ctx, 0, false, work, out, 1, int(shape[0]), inner, R(1), nullptr, 1, kOutLines);
+        else
+            s = line_fft<R>(ctx, 0, false, work, work, outer, int(shape[ax]), inner, R(1));
         if (s != NDC_OK) return s;
         inner *= shape[ax];
     }
-    if (N > 1) return transpose<R>(ctx, work, out, shape[0], rows / shape[0] * H);
+    if (N > 1 && !fused) return transpose<R>(ctx, work, out, shape[0], rows / shape[0] * H);
     return NDC_OK;
 }
 
@@ -2700,13 +2731,19 @@ ndc_status proc_backward(bool real, int N, const int64_t* sshape, int64_t fl, co
     if (N > 1) {
         work = static_cast<C*>(ctx.scratch(0, size_t(rows * H) * sizeof(C)));
         if (!work) return NDC_ERR_OUT_OF_MEMORY;
-        ndc_status s = transpose<R>(ctx, in, work, rows / nat[0] * H, nat[0]);
+        // the first (axis 0) transform reads the rotated layout itself when
+        // its lines fit LDS: line i of [F0][inner] is in[i F0 ...]
+        const bool fused = line_fits_lds<R>(ctx, 0, int(nat[0]));
+        ndc_status s = fused ? NDC_OK : transpose<R>(ctx, in, work, rows / nat[0] * H, nat[0]);
         if (s != NDC_OK) return s;
         int64_t outer = 1;
         for (int ax = 0; ax < N - 1; ++ax) {
             int64_t inner = H;
             for (int i = ax + 1; i < N - 1; ++i) inner *= nat[i];
-            s = line_fft<R>(ctx, 0, true, work, work, outer, int(nat[ax]), inner, R(1));
+            if (ax == 0 && fused)
+                s = line_fft<R>(ctx, 0, true, in, work, 1, int(nat[0]), inner, R(1), nullptr, 1, kInLines);
+            else
+                s = line_fft<R>(ctx, 0, true, work, work, outer, int(nat[ax]), inner, R(1));
             if (s != NDC_OK) return s;
             outer *= nat[ax];
         }
