@@ -20,7 +20,7 @@ TOL = {np.dtype(np.float32): 2e-6, np.dtype(np.float64): 1e-12, np.dtype(np.comp
        np.dtype(np.complex128): 1e-12}
 
 SHAPES = [(6,), (7,), (425,), (5120,), (4, 6), (9, 35), (128, 145), (18, 128, 288), (3, 5, 7), (2, 3, 4, 5),
-          (256, 5120)]
+          (256, 5120), (16, 1), (5, 1, 3), (1, 7)]
 
 
 def make(rng, shape, dt):
