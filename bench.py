@@ -10,7 +10,7 @@ PaddingMode::Circular) in one persistent launch (a HIP-graph replay of the
 same step is timed beside it), the autotuner choosing BF16x3 / BF16x6 matrix-core direct, FP32 direct or
 FP32 FFT per pair.  Other workloads (--workload): fft2d, fft2d_large, fft1d,
 fft3d, fft3d_c64, direct2d_i32, direct2d_f32 (with --algo bf16 / bf16x3), proc2d (the FFT
-processor's forward + backward round trip of a 256x5120 f32 array), copy.  The conv_fft
+processor's forward + backward round trip of a 256x5120 f32 array, --proc-shape for others), copy.  The conv_fft
 workloads take --baked (the kernel spectrum made once before timing).
 
 Prints ONE JSON line with the throughput, the error against the FP64 oracle
@@ -281,14 +281,16 @@ def batch_cpu_baseline(n, k, x, kv, threads, budget_s=10.0):
 
 def run_proc(args):
     """FftProcessor (processor/mod.rs:79-121, real.rs:105-155,233-281): one step is
-    forward then backward of a 256x5120 f32 array on the device, the processor
-    shape of the 2-D configuration."""
+    forward then backward of an f32 array on the device (--proc-shape, default
+    256x5120, the processor shape of the 2-D configuration)."""
     import torch
     import ndconv as nc
     from ndconv import _lib as C
     from oracle import ndconv_oracle as O
 
-    shape = (256, 5120)
+    shape = tuple(int(v) for v in args.proc_shape.split("x"))
+    if len(shape) != 2 or min(shape) < 1:
+        sys.exit("--proc-shape takes ROWSxCOLS")
     spec_shape = (shape[1] // 2 + 1, shape[0])
     x = np.random.default_rng(7).standard_normal(shape).astype(np.float32)
     dev = torch.device("cuda:0")
@@ -354,7 +356,8 @@ def run_proc(args):
     line = {"metric": METRIC, "value": outputs / (ms * 1e-3), "unit": "outputs/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "FftProcessor forward + backward, 256x5120 f32 (spectrum 2561x256 c64)",
+            "config": {"workload": f"FftProcessor forward + backward, {shape[0]}x{shape[1]} f32 "
+                                   f"(spectrum {spec_shape[0]}x{spec_shape[1]} c64)",
                        "outputs_per_step": outputs, "seed": 7},
             "forward_max_rel_err": fwd_err[0], "roundtrip_max_rel_err": rt_err[0], "roofline": roof,
             "cpu_baseline": cpu, "kernels": kernels, "wall_s": wall}
@@ -488,6 +491,7 @@ def main():
                     help="uneven batch: auto_bf16x3 (autotuned matrix-core/FP32 mix), auto (FP32 only), fft, direct, "
                          "bf16x3, bf16x6")
     ap.add_argument("--plain-steps", type=int, default=2, help="uneven batch: steps of the unbatched path")
+    ap.add_argument("--proc-shape", default="256x5120", help="proc2d: the f32 array's ROWSxCOLS")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU only (the library has no multi-GPU path); run with --gpus 1"}))
